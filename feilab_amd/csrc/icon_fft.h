@@ -1,0 +1,212 @@
+// Device-side FFT building blocks for CDNA4 (gfx950).
+//
+// A line of N complex values (N a power of two, 128..2048) is transformed by a
+// team of T = N/16 threads.  Each thread holds 16 values in registers.  The
+// transform is a Stockham autosort sequence of radix-16 passes followed by one
+// pass of radix N/16^k (2, 4, 8 or 16); each pass is an in-register DFT with
+// twiddles from a per-size table (double-precision sin/cos rounded to float)
+// and passes exchange data through LDS.  Unnormalised, sign DIR = -1 (forward,
+// e^{-2 pi i}) or +1 (inverse).
+//
+// Register order conventions (t = thread index within the team, T = N/16):
+//   input  : v[r]            holds element t + r*T                 (r < 16)
+//   output : v[q*RL + r]     holds element t + q*T + r*(N/RL)      (RL = last radix)
+// so both the first load and the last store of a team touch consecutive
+// addresses across consecutive threads.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace icon {
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// cos / sin of 2 pi q / 16, q = 0..7.
+constexpr float kCos16[8] = {1.0f, 0.923879532511286756f, 0.707106781186547524f,
+                                        0.382683432365089772f, 0.0f, -0.382683432365089772f,
+                                        -0.707106781186547524f, -0.923879532511286756f};
+constexpr float kSin16[8] = {0.0f, 0.382683432365089772f, 0.707106781186547524f,
+                                        0.923879532511286756f, 1.0f, 0.923879532511286756f,
+                                        0.707106781186547524f, 0.382683432365089772f};
+
+// a * exp(DIR * 2 pi i * Q / 16), Q in [0, 8), with the trivial cases exact.
+template <int Q, int DIR>
+__device__ __forceinline__ float2 rot16(float2 a) {
+  if constexpr (Q == 0) {
+    return a;
+  } else if constexpr (Q == 4) {
+    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+  } else {
+    constexpr float c = kCos16[Q];
+    constexpr float s = DIR > 0 ? kSin16[Q] : -kSin16[Q];
+    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+template <int LOG>
+__host__ __device__ constexpr int bitrev(int i) {
+  int r = 0;
+  for (int b = 0; b < LOG; b++) r |= ((i >> b) & 1) << (LOG - 1 - b);
+  return r;
+}
+
+template <int R> struct Log2 { static constexpr int v = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4; };
+
+// Stage of the radix-2 decimation-in-time network: butterflies of span LEN.
+template <int R, int DIR, int LEN, int K>
+__device__ __forceinline__ void dit_stage_k(float2* v) {
+  if constexpr (K < LEN / 2) {
+#pragma unroll
+    for (int i = K; i < R; i += LEN) {
+      float2 u = v[i];
+      float2 t = rot16<(16 / LEN) * K, DIR>(v[i + LEN / 2]);
+      v[i] = c_add(u, t);
+      v[i + LEN / 2] = c_sub(u, t);
+    }
+    dit_stage_k<R, DIR, LEN, K + 1>(v);
+  }
+}
+
+template <int R, int DIR, int LEN>
+__device__ __forceinline__ void dit_stages(float2* v) {
+  if constexpr (LEN <= R) {
+    dit_stage_k<R, DIR, LEN, 0>(v);
+    dit_stages<R, DIR, LEN * 2>(v);
+  }
+}
+
+// In-register DFT of size R (2, 4, 8, 16), natural order in and out.
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v) {
+  constexpr int L = Log2<R>::v;
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    const int j = bitrev<L>(i);
+    if (i < j) {
+      float2 t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+  dit_stages<R, DIR, 2>(v);
+}
+
+template <int N, int NS>
+__host__ __device__ constexpr int pass_radix() {
+  return (N / NS) >= 16 ? 16 : N / NS;
+}
+
+template <int N>
+__host__ __device__ constexpr int last_radix() {
+  int ns = 1;
+  int r = 16;
+  while (ns < N) {
+    r = (N / ns) >= 16 ? 16 : N / ns;
+    ns *= r;
+  }
+  return r;
+}
+
+// Loads a pass's inputs from LDS (element stride ES float2).
+template <int N, int RP, int ES>
+__device__ __forceinline__ void pass_load(float2 (&v)[16], const float2* lds, int t) {
+  constexpr int T = N / 16;
+  constexpr int B = 16 / RP;
+#pragma unroll
+  for (int q = 0; q < B; q++)
+#pragma unroll
+    for (int r = 0; r < RP; r++) v[q * RP + r] = lds[(t + q * T + r * (N / RP)) * ES];
+}
+
+// Twiddle and DFT of a pass with radix RP whose sub-transforms have size NS.
+template <int N, int RP, int NS, int DIR>
+__device__ __forceinline__ void pass_compute(float2 (&v)[16], int t, const float2* __restrict__ tw) {
+  constexpr int T = N / 16;
+  constexpr int B = 16 / RP;
+#pragma unroll
+  for (int q = 0; q < B; q++) {
+    if constexpr (NS > 1) {
+      const int j = t + q * T;
+      const int k = j & (NS - 1);
+      constexpr int step = N / (NS * RP);
+#pragma unroll
+      for (int r = 1; r < RP; r++) {
+        float2 w = tw[r * k * step];
+        if (DIR > 0) w.y = -w.y;
+        v[q * RP + r] = c_mul(v[q * RP + r], w);
+      }
+    }
+    dft<RP, DIR>(&v[q * RP]);
+  }
+}
+
+// Stores a pass's outputs to LDS in Stockham order.
+template <int N, int RP, int NS, int ES>
+__device__ __forceinline__ void pass_store(const float2 (&v)[16], float2* lds, int t) {
+  constexpr int T = N / 16;
+  constexpr int B = 16 / RP;
+#pragma unroll
+  for (int q = 0; q < B; q++) {
+    const int j = t + q * T;
+    const int k = j & (NS - 1);
+    const int base = (j - k) * RP + k;
+#pragma unroll
+    for (int r = 0; r < RP; r++) lds[(base + r * NS) * ES] = v[q * RP + r];
+  }
+}
+
+// Runs the passes from sub-transform size NS on; v holds that pass's inputs.
+// Every thread of the block must call this (it contains __syncthreads).
+template <int N, int ES, int DIR, int NS>
+__device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, const float2* __restrict__ tw) {
+  constexpr int RP = pass_radix<N, NS>();
+  pass_compute<N, RP, NS, DIR>(v, t, tw);
+  if constexpr (NS * RP < N) {
+    pass_store<N, RP, NS, ES>(v, lds, t);
+    __syncthreads();
+    constexpr int RN = pass_radix<N, NS * RP>();
+    pass_load<N, RN, ES>(v, lds, t);
+    __syncthreads();
+    fft_from<N, ES, DIR, NS * RP>(v, lds, t, tw);
+  }
+}
+
+// Registers in (input order) -> registers out (output order).  lds: N
+// float2 of scratch for this line at element stride ES.
+template <int N, int ES, int DIR>
+__device__ __forceinline__ void fft_regs(float2 (&v)[16], float2* lds, int t, const float2* __restrict__ tw) {
+  fft_from<N, ES, DIR, 1>(v, lds, t, tw);
+}
+
+// Element index held in output slot i of thread t.
+template <int N>
+__device__ __forceinline__ int out_index(int t, int i) {
+  constexpr int RL = last_radix<N>();
+  constexpr int T = N / 16;
+  return t + (i / RL) * T + (i % RL) * (N / RL);
+}
+
+// Element index held in input slot r of thread t.
+template <int N>
+__device__ __forceinline__ int in_index(int t, int r) {
+  return t + r * (N / 16);
+}
+
+// LDS line in natural order -> transformed line in natural order (in place).
+// Callers synchronise before (data present) and after (data consumed).
+template <int N, int ES, int DIR>
+__device__ __forceinline__ void fft_lds(float2* lds, int t, const float2* __restrict__ tw) {
+  float2 v[16];
+  pass_load<N, 16, ES>(v, lds, t);
+  __syncthreads();
+  fft_regs<N, ES, DIR>(v, lds, t, tw);
+  // the last pass_load is followed by a barrier inside fft_from, so the
+  // line can be overwritten now
+#pragma unroll
+  for (int i = 0; i < 16; i++) lds[out_index<N>(t, i) * ES] = v[i];
+}
+
+}  // namespace icon

@@ -1,0 +1,213 @@
+// Host-side geometry of an ICON plan: oversampling, Gaussian window tables,
+// deapodisation, radial nodes, density weights and the column-tile bin lists
+// used by the spread/interpolation kernels.
+//
+// The float/double placement follows the reference setup so that the device
+// tables equal the reference's (paths relative to
+// tomography/ICON_GPU_v1.2.9/sourceCode/ICON-GPU/):
+//   oversampling          MainCode.cu:63-71
+//   window b, bf          nufft_gpu_v8.cu:471-476, 1330-1331
+//   window taps           nufft_gpu_v8.cu:221-235 (2D), 1242-1256 (1D)
+//   deapodisation         nufft_gpu_v8.cu:22-42 (2D), 1258-1271 (1D)
+//   radial nodes          px2D.cu:8-47 (pi = 3.14159265, px2D.cuh:8)
+//   1D nodes              MainCode.cu:75-78
+//   density weights       weight.cu:8-41
+//   omitted angle         MainCode.cu:46-49
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "icon_plan.h"
+
+namespace icon {
+
+namespace {
+constexpr double kPI = 3.1415926535898;   // nufft_gpu_v8.cuh:8
+constexpr double kPiNodes = 3.14159265;   // px2D.cuh:8
+
+// b = 2 sigma m / ((2 sigma - 1) pi): float numerator, double denominator,
+// stored in float; bf = 1/sqrt(pi b) stored in float.
+void window_shape(float sigma, int m, float* b, float* bf) {
+  const float num = 2.0f * sigma * (float)m;
+  const double den = (double)(2.0f * sigma - 1.0f) * kPI;
+  *b = (float)((double)num / den);
+  *bf = (float)(1.0 / std::sqrt(kPI * (double)*b));
+}
+
+// One factor of the deapodisation exp(-b (pi k / n)^2): double exponent with
+// the float b and float n, stored in float.
+float deapod(int k, int N, float sigma, int m) {
+  const float n = std::ceil((float)N * sigma);
+  float b, bf;
+  window_shape(sigma, m, &b, &bf);
+  const double a = kPI * (double)k / (double)n;
+  return (float)std::exp(-(double)b * (a * a));
+}
+
+// Window taps of one coordinate (float arithmetic, expf).
+void taps(float c, int N, float sigma, float b, float bf, int m, float* phi, int* first) {
+  const int n = (int)std::ceil((float)N * sigma);
+  const float fl = std::floor((float)n * c);
+  for (int t = 0; t < 2 * m + 2; t++) {
+    const int l = (int)(fl - (float)m + (float)t);
+    const float x1 = c - (float)l / (float)n;
+    const float nx1 = (float)n * x1;
+    phi[t] = bf * std::exp(-(nx1 * nx1 / b));
+    if (t == 0) *first = (l + n) % n;
+  }
+}
+}  // namespace
+
+void grid_size(int N, float* sigma, int* n) {
+  const float uptop = (float)std::pow(2.0, std::ceil(std::log10((double)N) / std::log10(2.0)));
+  const float s = (float)(2.0 * uptop / (double)N);
+  *sigma = s;
+  *n = (int)std::ceil(s * (float)N);
+}
+
+bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err) {
+  if (N0 < 8 || N1 < 8 || nz < 2) {
+    if (err) *err = "icon_plan_create: need nx >= 8, thickness >= 8 and at least 2 tilts";
+    return false;
+  }
+  if (nz > 1000) {
+    if (err) *err = "icon_plan_create: more than 1000 tilts";
+    return false;
+  }
+  g.N0 = N0; g.N1 = N1; g.nz = nz; g.M = N0 * nz;
+  grid_size(N0, &g.sig0, &g.n0);
+  grid_size(N1, &g.sig1, &g.n1);
+  for (int n : {g.n0, g.n1}) {
+    if (n < 128 || n > 2048 || (n & (n - 1)) != 0) {
+      if (err) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf,
+                      "icon_plan_create: oversampled grid %d unsupported (widths and thickness 33..1024 "
+                      "give grids 128..2048)", n);
+        *err = buf;
+      }
+      return false;
+    }
+  }
+  window_shape(g.sig0, kWindowM, &g.b0, &g.bf0);
+  window_shape(g.sig1, kWindowM, &g.b1, &g.bf1);
+
+  // angles are negated on read (MainCode.cu:42)
+  g.theta.resize(nz);
+  for (int i = 0; i < nz; i++) g.theta[i] = -tilt_deg[i];
+  float om = g.theta[0];
+  for (int i = 0; i < nz; i++)
+    if (std::fabs(om) > std::fabs(g.theta[i])) om = g.theta[i];
+  g.omit = om;
+  g.iomit = 0;
+  for (int i = 0; i < nz; i++)
+    if (g.theta[i] == om) { g.iomit = i; break; }
+
+  // radial nodes: float accumulation of the radius, double trig
+  const int M = g.M;
+  g.nodes.resize(2 * (size_t)M);
+  {
+    std::vector<float> f(N0);
+    const float step = (float)(1.0 / N0);
+    f[0] = -0.5f;
+    for (int k = 1; k < N0; k++) f[k] = f[k - 1] + step;
+    for (int i = 0; i < nz; i++) {
+      const double a = (double)g.theta[i] * kPiNodes / 180.0;
+      const double c = std::cos(a), s = std::sin(a);
+      for (int k = 0; k < N0; k++) {
+        g.nodes[2 * ((size_t)i * N0 + k)] = (float)((double)f[k] * c);
+        g.nodes[2 * ((size_t)i * N0 + k) + 1] = (float)((double)f[k] * s);
+      }
+    }
+  }
+
+  // density weights: |ramp| x angular spacing, centre 180/(4 nz)
+  g.w.resize(M);
+  {
+    const int center = N0 / 2;
+    const float* th = g.theta.data();
+    for (int i = 0; i < nz; i++) {
+      const float span = (i == 0) ? (th[1] - th[0]) : (i == nz - 1) ? (th[nz - 2] - th[nz - 1])
+                                                                     : (th[i - 1] - th[i + 1]);
+      const double mult = (i == 0 || i == nz - 1) ? 2.0 : 1.0;
+      for (int k = 0; k < N0; k++) {
+        float v;
+        if (k == center) v = (float)(180.0 / (4 * nz));
+        else v = (float)(mult * std::fabs((double)(span * (float)(k - center))));
+        g.w[(size_t)i * N0 + k] = v;
+      }
+    }
+  }
+
+  // 2D window taps and first indices
+  g.phix.resize((size_t)M * kTaps);
+  g.phiy.resize((size_t)M * kTaps);
+  g.ix0.resize(M);
+  g.iy0.resize(M);
+  for (int j = 0; j < M; j++) {
+    taps(g.nodes[2 * (size_t)j], N0, g.sig0, g.b0, g.bf0, kWindowM, &g.phix[(size_t)j * kTaps], &g.ix0[j]);
+    taps(g.nodes[2 * (size_t)j + 1], N1, g.sig1, g.b1, g.bf1, kWindowM, &g.phiy[(size_t)j * kTaps], &g.iy0[j]);
+  }
+
+  // deapodisation
+  g.ck.resize((size_t)N0 * N1);
+  {
+    std::vector<float> cy(N1);
+    for (int Y = 0; Y < N1; Y++) cy[Y] = deapod(Y - N1 / 2, N1, g.sig1, kWindowM);
+    for (int X = 0; X < N0; X++) {
+      const float cx = deapod(X - N0 / 2, N0, g.sig0, kWindowM);
+      for (int Y = 0; Y < N1; Y++) g.ck[(size_t)X * N1 + Y] = cx * cy[Y];
+    }
+  }
+
+  // 1D plan of the projection rows: nodes (k - N0/2)/N0, same sigma as dim 0
+  g.ck1.resize(N0);
+  g.phi1.resize((size_t)N0 * kTaps);
+  g.i10.resize(N0);
+  for (int k = 0; k < N0; k++) {
+    const float x = (float)(((double)(float)k - (double)(float)N0 / 2.0) / (double)(float)N0);
+    taps(x, N0, g.sig0, g.b0, g.bf0, kWindowM, &g.phi1[(size_t)k * kTaps], &g.i10[k]);
+    g.ck1[k] = deapod(k - N0 / 2, N0, g.sig0, kWindowM);
+  }
+  return true;
+}
+
+// Bins every node's run of y-taps by grid-column tile.  A node's 14 taps are
+// consecutive columns (mod n1), so they fall into at most ceil(13/C)+1 tiles;
+// each (tile, node) pair is one entry recording the tap range, the node's
+// partial-sum slot and the first column inside the tile.  Entries of a tile
+// are in ascending node order.
+void build_bins(const Geometry& g, int C, Bins& b) {
+  b.C = C;
+  b.nblk = g.n1 / C;
+  b.off.assign(b.nblk + 1, 0);
+  b.nslot.assign(g.M, 0);
+  const int mask = g.n1 - 1;
+  // count
+  for (int j = 0; j < g.M; j++) {
+    int prev = -1;
+    for (int t = 0; t < kTaps; t++) {
+      const int blk = ((g.iy0[j] + t) & mask) / C;
+      if (blk != prev) { b.off[blk + 1]++; prev = blk; }
+    }
+  }
+  for (int i = 0; i < b.nblk; i++) b.off[i + 1] += b.off[i];
+  b.ent.resize(b.off[b.nblk]);
+  std::vector<int> fill(b.off.begin(), b.off.end() - 1);
+  for (int j = 0; j < g.M; j++) {
+    int t = 0, slot = 0;
+    while (t < kTaps) {
+      const int col = (g.iy0[j] + t) & mask;
+      const int blk = col / C;
+      int t1 = t;
+      while (t1 + 1 < kTaps && (((g.iy0[j] + t1 + 1) & mask) / C) == blk) t1++;
+      const int col0 = col - blk * C;
+      b.ent[fill[blk]++] = make_int2(j, t | (t1 << 8) | (slot << 16) | (col0 << 24));
+      slot++;
+      t = t1 + 1;
+    }
+    b.nslot[j] = (int8_t)slot;
+  }
+}
+
+}  // namespace icon

@@ -1,0 +1,106 @@
+// Kernel argument blocks and launchers of the ICON core (host + device view).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace icon {
+
+constexpr int kTapsK = 14;
+constexpr int kMaxSlotK = 3;
+
+struct RowFwdArgs {
+  int rows, N0, N1;
+  int cplx;        // 0: real image m (with optional update), 1: complex rhat
+  int update;      // 0 none, 1 INFR, 2 ICON (threshold)
+  int data_type;
+  float thr;
+  float* m;
+  const float2* rhat;
+  const double* alpha;
+  const float* ck;
+  const float2* tw;
+  float2* out;     // [S][N0][NR]
+};
+
+struct RowAdjArgs {
+  int rows, N0, N1;
+  int mode;            // 0: write image (A^H result), 1: rhat = image - ahwb, row sums
+  const float2* in;    // [S][N0][NR]
+  const float* ck;
+  const float2* tw;
+  float2* img;         // mode 0 output
+  const float2* ahwb;  // mode 1
+  float2* rhat;        // mode 1 output
+  double* rowpart;     // mode 1: sum |rhat|^2 per row
+};
+
+struct ColArgs {
+  int S, N0, N1, M, n1;
+  const float2* in;        // fwd: tmp [S][N0][n1]
+  float2* out;             // adj: tmp [S][N0][n1]
+  const float2* tw;
+  const int* ent_off;
+  const int2* ent;
+  const float* phix;
+  const float* phiy;
+  const int* ix0;
+  float2* part;            // fwd: [S][M][3]
+  const float2* fw;        // adj: [S][M]
+};
+
+struct CombineArgs {
+  int S, M, N0, nz;
+  int mode;              // 0: after A m, 1: after A rhat (step size), 2: initial b, 3: plain sum
+  const float2* part;    // [S][M][3]
+  const int8_t* nslot;
+  const float* w;
+  const float2* b;       // [S][M]
+  float2* fw;            // mode 0/2: weighted node values; mode 3: sums
+  double* bnorm;         // [S]
+  double* resid;         // mode 0: trace base or null
+  int resid_stride, resid_index;
+  const double* rowpart; // mode 1: [S][N0]
+  double* alpha;         // mode 1: [S]
+  double* alpha_trace;   // mode 1: [S][iters] or null
+  int alpha_stride, alpha_index;
+  float2* frep;          // mode 0: [S][N0] unweighted values at the omit tilt, or null
+  int iomit;
+};
+
+struct SinoArgs {
+  int rows, N0;
+  const float* sino;     // [S][nz][N0]
+  const float* ck1;
+  const float* phi1;
+  const int* i10;
+  const float2* tw;
+  float2* b;             // [S][nz*N0]
+};
+
+struct ReprojArgs {
+  int S, N0;
+  float sigma;
+  const float2* frep;    // [S][N0]
+  const float* ck1;
+  const float* phi1;
+  const int* i10;
+  const float2* tw;
+  float* out;            // [S][N0]
+};
+
+hipError_t launch_row_fwd(int NR, const RowFwdArgs& a, hipStream_t st);
+hipError_t launch_row_adj(int NR, const RowAdjArgs& a, hipStream_t st);
+hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t st);
+hipError_t launch_combine(const CombineArgs& a, hipStream_t st);
+hipError_t launch_sino(int NR, const SinoArgs& a, hipStream_t st);
+hipError_t launch_reproj(int NR, const ReprojArgs& a, hipStream_t st);
+hipError_t launch_fft_rows(int NR, float2* data, int rows, int sign, const float2* tw, hipStream_t st);
+hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int sign, const float2* tw,
+                           hipStream_t st);
+hipError_t launch_zero(float* p, size_t n, hipStream_t st);
+hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
+bool col_config_ok(int NC, int C);
+hipError_t set_col_attrs(int NC, int C);
+
+}  // namespace icon

@@ -1,0 +1,656 @@
+// CDNA4 (gfx950) kernels of the ICON reconstruction core.
+//
+// Data layout in HBM (S slices batched, complex = interleaved float2):
+//   m      [S][N0][N1] f32       image, index X*N1 + Y (X = width, Y = depth)
+//   rhat   [S][N0][N1] c64       gradient  A^H W (A m) - A^H W b
+//   ahwb   [S][N0][N1] c64       A^H W b
+//   tmp    [S][N0][n1] c64       the N0 non-zero grid rows after the y-pass
+//   part   [S][M][3]   c64       interpolation partials, one per column tile
+//   fw     [S][M]      c64       weighted node values (spread input)
+//   b      [S][M]      c64       Fourier-slice data
+// The oversampled grid (n0 x n1, x slow) never exists in HBM as a whole: the
+// forward transform writes only the N0 rows that the zero padding leaves
+// non-zero, and the column pass keeps its n0 x C tile in LDS where it is
+// interpolated from (forward) or spread into (adjoint).
+//
+// Reference kernels replaced (tomography/ICON_GPU_v1.2.9/sourceCode/ICON-GPU/):
+//   k_row_fwd      nufft_fhatDivCk_assignGhat(_R2C) :754-809 + y half of cufftExecC2C
+//                  (+ ICONUpdateCU_R / INFRUpdateCU_R :1597-1639 fused in front)
+//   k_col_fwd      x half of cufftExecC2C + nufft_GPhi_p1CU_noPreCompute :669-700
+//   k_combine      nufft_GPhi_p2CU :704-727, complexMulDoubleCU :1539-1549,
+//                  alpha1/matrixAdd/alpha2 :1652-1703 (FP64), data residual
+//   k_col_adj      cudaMemset + nufft_adjoint_fullpre_FPhi :1078-1102 + x half of
+//                  the inverse cufftExecC2C
+//   k_row_adj      y half of the inverse FFT + nufft_assignF_HAT :1104-1121 +
+//                  complexMinusCU :1563-1573 + alpha0 :1641-1650 (FP64)
+//   k_sino         nufft_fhatDivCk1d_assignGhat_1d :1432-1452 + 1D FFT +
+//                  nufft_GPhi_1dCU :1454-1477
+//   k_reproj       cunfft_adjoint_1d :1518-1536 (1D gather in ascending node
+//                  order, inverse FFT, nufft_assignF_HAT_1d :1123-1135)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "icon_fft.h"
+#include "icon_kernels.h"
+
+namespace icon {
+
+// ------------------------------------------------------------------ row pass
+// One grid row of length NR per team of NR/16 threads; RPB rows per block.
+
+
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
+  constexpr int T = NR / 16;
+  __shared__ float2 lds[RPB * NR];
+  const int team = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int row = blockIdx.x * RPB + team;
+  const bool valid = row < a.rows;
+  const int s = valid ? row / a.N0 : 0;
+  const int X = valid ? row % a.N0 : 0;
+  const int h1 = a.N1 >> 1;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int y = in_index<NR>(t, r);
+    const int Y = (y + h1) & (NR - 1);
+    float2 gv = make_float2(0.f, 0.f);
+    if (valid && Y < a.N1) {
+      const size_t idx = ((size_t)s * a.N0 + X) * a.N1 + Y;
+      const float c = a.ck[(size_t)X * a.N1 + Y];
+      if (!a.cplx) {
+        float mv = a.m[idx];
+        if (a.update) {
+          // m - al*Re(rhat) in double, stored as float, then the threshold
+          const double al = a.alpha[s];
+          float nv = (float)__fma_rn(-al, (double)a.rhat[idx].x, (double)mv);
+          if (a.update == 2) {
+            if (a.data_type == 1) { if (nv > a.thr) nv = a.thr; }
+            else { if (nv < a.thr) nv = a.thr; }
+          }
+          a.m[idx] = nv;
+          mv = nv;
+        }
+        gv.x = mv / c;
+      } else {
+        const float2 rv = a.rhat[idx];
+        gv.x = rv.x / c;
+        gv.y = rv.y / c;
+      }
+    }
+    v[r] = gv;
+  }
+  fft_regs<NR, 1, -1>(v, lds + team * NR, t, a.tw);
+  if (valid) {
+    float2* o = a.out + (size_t)row * NR;
+#pragma unroll
+    for (int i = 0; i < 16; i++) o[out_index<NR>(t, i)] = v[i];
+  }
+}
+
+
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
+  constexpr int T = NR / 16;
+  __shared__ float2 lds[RPB * NR];
+  __shared__ double red[RPB * T];
+  const int team = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int row = blockIdx.x * RPB + team;
+  const bool valid = row < a.rows;
+  const int s = valid ? row / a.N0 : 0;
+  const int X = valid ? row % a.N0 : 0;
+  float2 v[16];
+  const float2* src = a.in + (size_t)(valid ? row : 0) * NR;
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = valid ? src[in_index<NR>(t, r)] : make_float2(0.f, 0.f);
+  fft_regs<NR, 1, 1>(v, lds + team * NR, t, a.tw);
+  const int h1 = a.N1 >> 1;
+  double acc = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int y = out_index<NR>(t, i);
+      const int Y = (y + h1) & (NR - 1);
+      if (Y < a.N1) {
+        const size_t idx = ((size_t)s * a.N0 + X) * a.N1 + Y;
+        const float c = a.ck[(size_t)X * a.N1 + Y];
+        const float2 f = make_float2(v[i].x / c, v[i].y / c);
+        if (a.mode == 0) {
+          a.img[idx] = f;
+        } else {
+          const float2 w = a.ahwb[idx];
+          const float2 r = make_float2(f.x - w.x, f.y - w.y);
+          a.rhat[idx] = r;
+          acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+        }
+      }
+    }
+  }
+  if (a.mode == 1) {
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (valid && t == 0) {
+      double sum = 0.0;
+      for (int i = 0; i < T; i++) sum += red[team * T + i];
+      a.rowpart[row] = sum;
+    }
+  }
+}
+
+// --------------------------------------------------------------- column pass
+// One block per (slice, tile of C grid columns); the n0 x C tile lives in LDS
+// (row-major, C fastest).  Thread (c = tid % C, t = tid / C) works on column c.
+
+
+template <int NC, int C>
+__global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
+  constexpr int NT = C * (NC / 16);
+  extern __shared__ float2 tile[];   // NC * C
+  const int nblk = a.n1 / C;
+  const int s = blockIdx.x / nblk;
+  const int blk = blockIdx.x % nblk;
+  const int y0 = blk * C;
+  const int tid = threadIdx.x;
+  const int h0 = a.N0 >> 1;
+  // load the non-zero rows (zero padding elsewhere)
+  for (int i = tid; i < NC * C; i += NT) {
+    const int x = i / C, c = i % C;
+    const int X = (x + h0) & (NC - 1);
+    tile[i] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<NC, C, -1>(tile + (tid % C), tid / C, a.tw);
+  __syncthreads();
+  // interpolation partials of this tile's nodes
+  const int e1 = a.ent_off[blk + 1];
+  for (int e = a.ent_off[blk] + tid; e < e1; e += NT) {
+    const int2 en = a.ent[e];
+    const int j = en.x;
+    const int t0 = en.y & 0xff, t1 = (en.y >> 8) & 0xff, slot = (en.y >> 16) & 0xff;
+    const int col0 = (en.y >> 24) & 0xff;
+    const int ixs = a.ix0[j];
+    float px[kTapsK];
+#pragma unroll
+    for (int tx = 0; tx < kTapsK; tx++) px[tx] = a.phix[(size_t)j * kTapsK + tx];
+    float sx = 0.f, sy = 0.f;
+    for (int ty = t0; ty <= t1; ty++) {
+      const float py = a.phiy[(size_t)j * kTapsK + ty];
+      const int c = col0 + ty - t0;
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int tx = 0; tx < kTapsK; tx++) {
+        const int x = (ixs + tx) & (NC - 1);
+        const float phi = px[tx] * py;
+        const float2 gv = tile[x * C + c];
+        ax = __fmaf_rn(gv.x, phi, ax);
+        ay = __fmaf_rn(gv.y, phi, ay);
+      }
+      sx += ax;
+      sy += ay;
+    }
+    a.part[((size_t)s * a.M + j) * kMaxSlotK + slot] = make_float2(sx, sy);
+  }
+}
+
+template <int NC, int C>
+__global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
+  constexpr int NT = C * (NC / 16);
+  extern __shared__ float2 tile[];   // NC * C
+  const int nblk = a.n1 / C;
+  const int s = blockIdx.x / nblk;
+  const int blk = blockIdx.x % nblk;
+  const int y0 = blk * C;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < NC * C; i += NT) tile[i] = make_float2(0.f, 0.f);
+  __syncthreads();
+  // spread this tile's nodes into LDS
+  float* tf = reinterpret_cast<float*>(tile);
+  const int e1 = a.ent_off[blk + 1];
+  for (int e = a.ent_off[blk] + tid; e < e1; e += NT) {
+    const int2 en = a.ent[e];
+    const int j = en.x;
+    const int t0 = en.y & 0xff, t1 = (en.y >> 8) & 0xff;
+    const int col0 = (en.y >> 24) & 0xff;
+    const int ixs = a.ix0[j];
+    const float2 f = a.fw[(size_t)s * a.M + j];
+    float px[kTapsK];
+#pragma unroll
+    for (int tx = 0; tx < kTapsK; tx++) px[tx] = a.phix[(size_t)j * kTapsK + tx];
+    for (int ty = t0; ty <= t1; ty++) {
+      const float py = a.phiy[(size_t)j * kTapsK + ty];
+      const int c = col0 + ty - t0;
+#pragma unroll
+      for (int tx = 0; tx < kTapsK; tx++) {
+        const int x = (ixs + tx) & (NC - 1);
+        const float phi = px[tx] * py;
+        atomicAdd(&tf[2 * (x * C + c)], f.x * phi);
+        atomicAdd(&tf[2 * (x * C + c) + 1], f.y * phi);
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<NC, C, 1>(tile + (tid % C), tid / C, a.tw);
+  __syncthreads();
+  // keep the N0 rows the crop needs
+  const int h0 = a.N0 >> 1;
+  for (int i = tid; i < a.N0 * C; i += NT) {
+    const int X = i / C, c = i % C;
+    const int x = (X - h0) & (NC - 1);
+    a.out[((size_t)s * a.N0 + X) * a.n1 + y0 + c] = tile[x * C + c];
+  }
+}
+
+// ------------------------------------------------------------ node combine
+// One block (1024 threads) per slice.  FP64 block reductions in a fixed order.
+
+
+__device__ __forceinline__ double block_sum_1024(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int h = 512; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void k_combine(CombineArgs a) {
+  __shared__ double red[1024];
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  double acc = 0.0;
+  const float2* P = a.part + (size_t)s * a.M * kMaxSlotK;
+  for (int j = tid; j < a.M; j += 1024) {
+    float2 f;
+    if (a.mode == 2) {
+      f = a.b[(size_t)s * a.M + j];
+    } else {
+      const int ns = a.nslot[j];
+      f = P[(size_t)j * kMaxSlotK];
+      for (int q = 1; q < ns; q++) {
+        const float2 p = P[(size_t)j * kMaxSlotK + q];
+        f.x += p.x;
+        f.y += p.y;
+      }
+    }
+    const float wj = a.w[j];
+    if (a.mode == 0) {
+      if (a.resid) {
+        const float2 bj = a.b[(size_t)s * a.M + j];
+        const double dx = (double)f.x - (double)bj.x, dy = (double)f.y - (double)bj.y;
+        acc += (double)wj * (dx * dx + dy * dy);
+      }
+      if (a.frep) {
+        const int i = j / a.N0;
+        if (i == a.iomit) a.frep[(size_t)s * a.N0 + (j - i * a.N0)] = f;
+      }
+      a.fw[(size_t)s * a.M + j] = make_float2(f.x * wj, f.y * wj);
+    } else if (a.mode == 1) {
+      acc += (double)f.x * (double)f.x * (double)wj + (double)f.y * (double)f.y * (double)wj;
+    } else if (a.mode == 2) {
+      acc += (double)wj * ((double)f.x * (double)f.x + (double)f.y * (double)f.y);
+      a.fw[(size_t)s * a.M + j] = make_float2(f.x * wj, f.y * wj);
+    } else {
+      a.fw[(size_t)s * a.M + j] = f;
+    }
+  }
+  if (a.mode == 3) return;
+  if (a.mode == 0 && !a.resid) return;
+  const double tot = block_sum_1024(acc, red);
+  if (a.mode == 0) {
+    if (tid == 0) {
+      const double bn = a.bnorm[s];
+      a.resid[(size_t)s * a.resid_stride + a.resid_index] = bn > 0 ? sqrt(tot / bn) : 0.0;
+    }
+  } else if (a.mode == 2) {
+    if (tid == 0) a.bnorm[s] = tot;
+  } else {
+    // rtr = sum of the row partials in row order
+    double rtr = 0.0;
+    if (tid == 0) {
+      for (int X = 0; X < a.N0; X++) rtr += a.rowpart[(size_t)s * a.N0 + X];
+      const double al = tot != 0.0 ? rtr / tot : 0.0;
+      a.alpha[s] = al;
+      if (a.alpha_trace) a.alpha_trace[(size_t)s * a.alpha_stride + a.alpha_index] = al;
+    }
+  }
+}
+
+// ------------------------------------------------------- projection rows (1D)
+// One team of NR/16 threads per (slice, tilt) row; RPB rows per block.
+
+
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
+  constexpr int T = NR / 16;
+  __shared__ float2 lds[RPB * NR];
+  const int team = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int row = blockIdx.x * RPB + team;
+  const bool valid = row < a.rows;
+  const int h = a.N0 >> 1;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int x = in_index<NR>(t, r);
+    const int X = (x + h) & (NR - 1);
+    float gv = 0.f;
+    if (valid && X < a.N0) gv = a.sino[(size_t)row * a.N0 + X] / a.ck1[X];
+    v[r] = make_float2(gv, 0.f);
+  }
+  float2* L = lds + team * NR;
+  fft_regs<NR, 1, -1>(v, L, t, a.tw);
+#pragma unroll
+  for (int i = 0; i < 16; i++) L[out_index<NR>(t, i)] = v[i];
+  __syncthreads();
+  if (valid) {
+    for (int k = t; k < a.N0; k += T) {
+      float ax = 0.f, ay = 0.f;
+      const int i0 = a.i10[k];
+#pragma unroll
+      for (int q = 0; q < kTapsK; q++) {
+        const float ph = a.phi1[(size_t)k * kTapsK + q];
+        const float2 gv = L[(i0 + q) & (NR - 1)];
+        ax = __fmaf_rn(gv.x, ph, ax);
+        ay = __fmaf_rn(gv.y, ph, ay);
+      }
+      a.b[(size_t)row * a.N0 + k] = make_float2(ax, ay);
+    }
+  }
+}
+
+// --------------------------------------------------------- reprojection (1D)
+// One team of NR/16 threads per slice.  The 1D spread is a gather per grid
+// cell over the nodes that touch it, in ascending node order.
+
+
+template <int NR>
+__global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
+  constexpr int T = NR / 16;
+  __shared__ float2 lds[NR];
+  const int s = blockIdx.x;
+  const int t = threadIdx.x;
+  const float2* f = a.frep + (size_t)s * a.N0;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int x = in_index<NR>(t, r);
+    float ax = 0.f, ay = 0.f;
+    // Nodes touching cell x sit within a few cells of x (unwrapped as x - NR
+    // for nodes near k = 0, as x otherwise); node spacing is sigma >= 2 cells,
+    // so +-10 nodes around each candidate covers every tap.  The two windows
+    // are visited in ascending k without overlap; membership is the exact
+    // wrapped-tap test.
+    int kprev = -1;
+    for (int pass = 0; pass < 2; pass++) {
+      const int u = pass == 0 ? x - NR : x;
+      const int kc = (int)floorf((float)u / a.sigma) + (a.N0 >> 1);
+      const int klo = max(max(kc - 10, 0), kprev + 1), khi = min(kc + 10, a.N0 - 1);
+      for (int k = klo; k <= khi; k++) {
+        const int d = (x - a.i10[k]) & (NR - 1);
+        if (d < kTapsK) {
+          const float ph = a.phi1[(size_t)k * kTapsK + d];
+          ax = __fmaf_rn(f[k].x, ph, ax);
+          ay = __fmaf_rn(f[k].y, ph, ay);
+        }
+      }
+      kprev = max(kprev, khi);
+    }
+    v[r] = make_float2(ax, ay);
+  }
+  fft_regs<NR, 1, 1>(v, lds, t, a.tw);
+  const int h = a.N0 >> 1;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int y = out_index<NR>(t, i);
+    const int X = (y + h) & (NR - 1);
+    if (X < a.N0) a.out[(size_t)s * a.N0 + X] = v[i].x / a.ck1[X];
+  }
+}
+
+// -------------------------------------------------------------- plain FFTs
+
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_fft_rows(float2* data, int rows, int sign,
+                                                             const float2* tw) {
+  constexpr int T = NR / 16;
+  __shared__ float2 lds[RPB * NR];
+  const int team = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int row = blockIdx.x * RPB + team;
+  const bool valid = row < rows;
+  float2* p = data + (size_t)(valid ? row : 0) * NR;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = valid ? p[in_index<NR>(t, r)] : make_float2(0.f, 0.f);
+  if (sign < 0) fft_regs<NR, 1, -1>(v, lds + team * NR, t, tw);
+  else fft_regs<NR, 1, 1>(v, lds + team * NR, t, tw);
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) p[out_index<NR>(t, i)] = v[i];
+  }
+}
+
+template <int NC, int C>
+__global__ __launch_bounds__(C * (NC / 16)) void k_fft_cols(float2* data, int n1, int count, int sign,
+                                                            const float2* tw) {
+  constexpr int NT = C * (NC / 16);
+  extern __shared__ float2 tile[];
+  const int nblk = n1 / C;
+  const int g = blockIdx.x / nblk;
+  const int y0 = (blockIdx.x % nblk) * C;
+  float2* base = data + (size_t)g * NC * n1;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < NC * C; i += NT) tile[i] = base[(size_t)(i / C) * n1 + y0 + i % C];
+  __syncthreads();
+  if (sign < 0) fft_lds<NC, C, -1>(tile + (tid % C), tid / C, tw);
+  else fft_lds<NC, C, 1>(tile + (tid % C), tid / C, tw);
+  __syncthreads();
+  for (int i = tid; i < NC * C; i += NT) base[(size_t)(i / C) * n1 + y0 + i % C] = tile[i];
+  (void)count;
+}
+
+__global__ void k_zero_f32(float* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+__global__ void k_copy_f32(float* dst, const float* src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// ================================================================ launchers
+
+namespace {
+
+template <int NR>
+constexpr int rows_per_block() {
+  return NR >= 2048 ? 2 : 256 / (NR / 16) > 16 ? 16 : 256 / (NR / 16);
+}
+
+}  // namespace
+
+#define ROW_CASES(X) X(128) X(256) X(512) X(1024) X(2048)
+
+hipError_t launch_row_fwd(int NR, const RowFwdArgs& a, hipStream_t st) {
+  const RowFwdArgs& h = a;
+  switch (NR) {
+#define CASE(N)                                                                                     \
+  case N: {                                                                                         \
+    constexpr int RPB = rows_per_block<N>();                                                        \
+    hipLaunchKernelGGL((k_row_fwd<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+                       st, a);                                                                      \
+    break;                                                                                          \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_row_adj(int NR, const RowAdjArgs& a, hipStream_t st) {
+  const RowAdjArgs& h = a;
+  switch (NR) {
+#define CASE(N)                                                                                     \
+  case N: {                                                                                         \
+    constexpr int RPB = rows_per_block<N>();                                                        \
+    hipLaunchKernelGGL((k_row_adj<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+                       st, a);                                                                      \
+    break;                                                                                          \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sino(int NR, const SinoArgs& a, hipStream_t st) {
+  const SinoArgs& h = a;
+  switch (NR) {
+#define CASE(N)                                                                                  \
+  case N: {                                                                                      \
+    constexpr int RPB = rows_per_block<N>();                                                     \
+    hipLaunchKernelGGL((k_sino<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+                       st, a);                                                                   \
+    break;                                                                                       \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_reproj(int NR, const ReprojArgs& a, hipStream_t st) {
+  const ReprojArgs& h = a;
+  switch (NR) {
+#define CASE(N)                                                                   \
+  case N:                                                                         \
+    hipLaunchKernelGGL((k_reproj<N>), dim3(h.S), dim3(N / 16), 0, st, a);         \
+    break;
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_rows(int NR, float2* data, int rows, int sign, const float2* tw, hipStream_t st) {
+  switch (NR) {
+#define CASE(N)                                                                                     \
+  case N: {                                                                                         \
+    constexpr int RPB = rows_per_block<N>();                                                        \
+    hipLaunchKernelGGL((k_fft_rows<N, RPB>), dim3((rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+                       st, data, rows, sign, tw);                                                   \
+    break;                                                                                          \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Column kernels: (NC, C) with NC * C * 8 bytes of LDS.
+#define COL_CASES(X) X(128, 16) X(128, 32) X(256, 8) X(256, 16) X(256, 32) X(512, 8) X(512, 16) X(512, 32) X(1024, 8) X(1024, 16) X(2048, 8)
+
+bool col_config_ok(int NC, int C) {
+#define CASE(N, CC) if (NC == N && C == CC) return true;
+  COL_CASES(CASE)
+#undef CASE
+  return false;
+}
+
+template <int NC, int C>
+static hipError_t col_attr() {
+  static bool done = false;
+  if (!done) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       NC * C * (int)sizeof(float2));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_col_adj<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            NC * C * (int)sizeof(float2));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_fft_cols<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            NC * C * (int)sizeof(float2));
+    if (e != hipSuccess) return e;
+    done = true;
+  }
+  return hipSuccess;
+}
+
+hipError_t set_col_attrs(int NC, int C) {
+#define CASE(N, CC) \
+  if (NC == N && C == CC) return col_attr<N, CC>();
+  COL_CASES(CASE)
+#undef CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t st) {
+  const ColArgs& h = a;
+  const int nblk = h.n1 / C;
+  hipError_t e = hipErrorInvalidValue;
+#define CASE(N, CC)                                                                                      \
+  if (NC == N && C == CC) {                                                                              \
+    e = col_attr<N, CC>();                                                                               \
+    if (e != hipSuccess) return e;                                                                       \
+    if (adjoint)                                                                                         \
+      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * CC * sizeof(float2), \
+                         st, a);                                                                         \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * CC * sizeof(float2), \
+                         st, a);                                                                         \
+    return hipGetLastError();                                                                            \
+  }
+  COL_CASES(CASE)
+#undef CASE
+  return e;
+}
+
+hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int sign, const float2* tw,
+                           hipStream_t st) {
+  const int nblk = n1 / C;
+#define CASE(N, CC)                                                                                     \
+  if (NC == N && C == CC) {                                                                             \
+    hipError_t e = col_attr<N, CC>();                                                                   \
+    if (e != hipSuccess) return e;                                                                      \
+    hipLaunchKernelGGL((k_fft_cols<N, CC>), dim3(count * nblk), dim3(CC * (N / 16)), N * CC * sizeof(float2), \
+                       st, data, n1, count, sign, tw);                                                  \
+    return hipGetLastError();                                                                           \
+  }
+  COL_CASES(CASE)
+#undef CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_combine(const CombineArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_combine, dim3(a.S), dim3(1024), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(float* p, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero_f32, dim3(blocks), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_copy_f32, dim3(blocks), dim3(256), 0, st, dst, src, n);
+  return hipGetLastError();
+}
+
+}  // namespace icon

@@ -1,0 +1,185 @@
+// File-level drop-in of ICON-GPU's per-device worker Lib_2D2
+// (tomography/ICON_GPU_v1.2.9/sourceCode/ICON-GPU/MainCode.cu:3-177):
+// angles -> plan -> for each slice: read the sinogram row of every tilt,
+// solve, write <result_dir>/midYYYYY.mrc and the reprojection row.
+// Differences: slices are solved in batches on the GPU, errors are returned
+// as status codes instead of exit(1), and the tilt count must match the
+// stack's nz.
+#include <chrono>
+#include <cmath>
+#include <mutex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/icon_mi355x.h"
+#include "mrc_io.h"
+
+
+namespace {
+std::mutex g_lmu;
+std::string g_lerr_store;
+void set_err(const std::string& s) {
+  std::lock_guard<std::mutex> lk(g_lmu);
+  g_lerr_store = s;
+  std::fprintf(stderr, "icon_lib_2d2: %s\n", s.c_str());
+}
+}  // namespace
+
+
+extern "C" int icon_read_tilts(const char* path, float* tilt_deg, int cap, int* count) {
+  // MainCode.cu:38-45: whitespace-separated degrees (negation happens in the plan)
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) return ICON_ERR_IO;
+  int n = 0;
+  float v;
+  while (std::fscanf(fp, "%f", &v) == 1) {
+    if (n < cap) tilt_deg[n] = v;
+    n++;
+  }
+  std::fclose(fp);
+  *count = n;
+  return n > cap ? ICON_ERR_CAPACITY : ICON_OK;
+}
+
+extern "C" const char* icon_lib_2d2_error(void) {
+  std::lock_guard<std::mutex> lk(g_lmu);
+  return g_lerr_store.c_str();
+}
+
+// Slice partition across devices (ICONGPU.cu:399-453): shares proportional to
+// the compute-unit count (floored), leftovers handed out round-robin starting
+// with the device of most units; the last device takes whatever remains.
+extern "C" int icon_partition_slices(int ndev, const int* cus, int y_start, int y_end, int* first, int* last) {
+  const int total = y_end - y_start + 1;
+  if (ndev < 1 || total < 1) return ICON_ERR_ARG;
+  std::vector<int> rate(ndev, 0), order(ndev, 0), taken(ndev, 0);
+  for (int d = 0; d < ndev; d++) {
+    int best = -1;
+    for (int i = 0; i < ndev; i++)
+      if (!taken[i] && (best < 0 || cus[best] < cus[i])) best = i;
+    order[d] = best;
+    taken[best] = 1;
+  }
+  double sum = 0;
+  for (int d = 0; d < ndev; d++) sum += cus[d];
+  for (int d = 0; d < ndev; d++) rate[d] = (int)std::floor((double)cus[d] / sum * total);
+  int assigned = 0;
+  for (int d = 0; d < ndev; d++) assigned += rate[d];
+  for (int i = 0, j = 0; i < total - assigned; i++) {
+    rate[order[j]]++;
+    j = (j + 1) % ndev;
+  }
+  const int used = ndev < total ? ndev : total;
+  int next = 0;
+  for (int d = 0; d < used; d++) {
+    int s, e;
+    if (d != ndev - 1) {
+      s = next;
+      e = rate[d] != 0 ? s + rate[d] - 1 : s;
+      next = e + 1;
+    } else {
+      s = next;
+      e = y_end - y_start;
+    }
+    first[d] = s + y_start;
+    last[d] = e + y_start;
+  }
+  return used;
+}
+
+
+
+extern "C" int icon_lib_2d2(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
+                            float* reproj, const char* angle_path, const char* result_dir, const int* before,
+                            const float* betain, int batch) {
+  if (!mrc_path || !angle_path || !result_dir || !before || !betain) {
+    set_err("icon_lib_2d2: null argument");
+    return ICON_ERR_ARG;
+  }
+  FILE* fin = std::fopen(mrc_path, "rb");
+  if (!fin) {
+    set_err(std::string("cannot open tilt series ") + mrc_path);
+    return ICON_ERR_IO;
+  }
+  MrcHeader head;
+  std::string err;
+  if (!mrc::read_header(fin, &head)) {
+    std::fclose(fin);
+    set_err("cannot read MRC header");
+    return ICON_ERR_IO;
+  }
+  const int nx = head.nx, nz = head.nz;
+  std::vector<float> tilts(1000);
+  int ntilt = 0;
+  int rc = icon_read_tilts(angle_path, tilts.data(), (int)tilts.size(), &ntilt);
+  if (rc == ICON_ERR_IO) {
+    std::fclose(fin);
+    set_err(std::string("Can not open anglefile '") + angle_path + "'!");
+    return ICON_ERR_IO;
+  }
+  if (ntilt < nz) {
+    std::fclose(fin);
+    set_err("tilt file has fewer angles than the stack has sections");
+    return ICON_ERR_ARG;
+  }
+  const int pre = before[0], post = before[1], data_type = before[3];
+  const float thr = betain[0];
+  const int thickness = (int)betain[1] > 0 ? (int)betain[1] : nx;
+  const int icon = iternum - pre - post;
+  if (icon < 0) {
+    std::fclose(fin);
+    set_err("iteration split exceeds the iteration count");
+    return ICON_ERR_ARG;
+  }
+  if (slice_end > head.ny) slice_end = head.ny;
+  const int total = slice_end - slice_begin;
+  if (total <= 0) {
+    std::fclose(fin);
+    return ICON_OK;
+  }
+  const int B = batch > 0 ? (batch < total ? batch : total) : (total < 256 ? total : 256);
+  icon_plan* plan = nullptr;
+  rc = icon_plan_create(&plan, device, nx, thickness, nz, tilts.data(), B);
+  if (rc) {
+    std::fclose(fin);
+    set_err(icon_last_error());
+    return rc;
+  }
+  icon_solve_opts o{};
+  o.pre = pre; o.icon = icon; o.post = post; o.data_type = data_type; o.threshold = thr; o.use_graph = 0;
+  std::vector<float> sino((size_t)B * nz * nx), m((size_t)B * nx * thickness), rp((size_t)B * nx);
+  for (int y0 = slice_begin; y0 < slice_end; y0 += B) {
+    const int cnt = (slice_end - y0) < B ? (slice_end - y0) : B;
+    if (!mrc::read_y_slices(fin, head, y0, cnt, sino.data(), &err)) {
+      rc = ICON_ERR_IO;
+      set_err(err);
+      break;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    rc = icon_reconstruct_host(plan, cnt, sino.data(), m.data(), nullptr, rp.data(), nullptr, nullptr, &o);
+    if (rc) {
+      set_err(icon_last_error());
+      break;
+    }
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int s = 0; s < cnt; s++) {
+      const int Y = y0 + s;
+      std::printf("slice %d finished, time %lf\n", Y, dt / cnt);
+      char path[4096];
+      std::snprintf(path, sizeof path, "%s/mid%05d.mrc", result_dir, Y);
+      if (!mrc::write_icon_slice(path, head, m.data() + (size_t)s * nx * thickness, nx, thickness, &err)) {
+        rc = ICON_ERR_IO;
+        set_err(err);
+        break;
+      }
+      if (reproj) std::memcpy(reproj + (size_t)(Y - slice_begin) * nx, rp.data() + (size_t)s * nx, sizeof(float) * nx);
+    }
+    if (rc) break;
+  }
+  std::printf("full reconstruction\n");
+  icon_plan_destroy(plan);
+  std::fclose(fin);
+  return rc;
+}

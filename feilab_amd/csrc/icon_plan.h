@@ -1,0 +1,104 @@
+// Internal plan state of the MI355X ICON core (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/icon_mi355x.h"
+
+namespace icon {
+
+constexpr int kTaps = 14;      // 2m+2, m = 6 (nufft_gpu_v8.cuh:5)
+constexpr int kWindowM = 6;
+constexpr int kMaxSlot = 3;    // node y-taps split over at most 3 column tiles (tile >= 8)
+
+// Host-side geometry of one tilt series, computed exactly as the reference's
+// setup does (float/double placement included).
+struct Geometry {
+  int N0 = 0, N1 = 0, nz = 0, M = 0, n0 = 0, n1 = 0;
+  float sig0 = 0, sig1 = 0, b0 = 0, b1 = 0, bf0 = 0, bf1 = 0;
+  int iomit = 0;
+  float omit = 0;
+  std::vector<float> theta;       // negated angles (MainCode.cu:42)
+  std::vector<float> nodes;       // 2*M, (x, y) per node j = i*N0 + k
+  std::vector<float> w;           // M density weights
+  std::vector<float> ck;          // N0*N1 deapodisation, index X*N1+Y
+  std::vector<float> ck1;         // N0
+  std::vector<float> phix, phiy;  // M*14 window taps
+  std::vector<int> ix0, iy0;      // M: wrapped first tap index
+  std::vector<float> phi1;        // N0*14 (1D plan)
+  std::vector<int> i10;           // N0
+};
+
+// Column-tile bin lists for the spread / interpolation.
+struct Bins {
+  int C = 16;                     // grid columns per tile
+  int nblk = 0;                   // n1 / C
+  std::vector<int> off;           // nblk+1
+  std::vector<int2> ent;          // (node j, t0 | t1<<8 | slot<<16 | col0<<24)
+  std::vector<int8_t> nslot;      // M
+};
+
+// Fills g; returns false (with a message) on an invalid geometry.
+bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err);
+void build_bins(const Geometry& g, int C, Bins& b);
+void grid_size(int N, float* sigma, int* n);
+
+struct GraphKey {
+  int S = -1;
+  const void* sino = nullptr;
+  void* m = nullptr;
+  const void* m_init = nullptr;
+  void* reproj = nullptr;
+  void* resid = nullptr;
+  void* alpha = nullptr;
+  int pre = 0, icon = 0, post = 0, data_type = 0;
+  float thr = 0;
+  bool operator==(const GraphKey& o) const {
+    return S == o.S && sino == o.sino && m == o.m && m_init == o.m_init && reproj == o.reproj &&
+           resid == o.resid && alpha == o.alpha && pre == o.pre && icon == o.icon &&
+           post == o.post && data_type == o.data_type && thr == o.thr;
+  }
+};
+
+}  // namespace icon
+
+// Opaque plan of the C ABI.
+struct icon_plan {
+  int device = 0;
+  icon::Geometry g;
+  icon::Bins bins;
+  int S_max = 0;
+  hipStream_t stream = nullptr;     // plan-owned stream (capturable)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // shared tables
+  float* d_ck = nullptr;
+  float* d_ck1 = nullptr;
+  float* d_w = nullptr;
+  float* d_phix = nullptr;
+  float* d_phiy = nullptr;
+  int* d_ix0 = nullptr;
+  int* d_ent_off = nullptr;
+  int2* d_ent = nullptr;
+  int8_t* d_nslot = nullptr;
+  float* d_phi1 = nullptr;
+  int* d_i10 = nullptr;
+  const float2* d_tw0 = nullptr;    // twiddles for n0 (shared cache)
+  const float2* d_tw1 = nullptr;    // twiddles for n1
+  // per-slice work buffers (capacity S_max)
+  float2* d_b = nullptr;            // [S][M] Fourier-slice data
+  float2* d_fw = nullptr;           // [S][M] weighted node values
+  float2* d_part = nullptr;         // [S][M][kMaxSlot] interpolation partials
+  float2* d_ahwb = nullptr;         // [S][N0][N1]
+  float2* d_rhat = nullptr;         // [S][N0][N1]
+  float2* d_tmp = nullptr;          // [S][N0][n1] row-transform intermediate
+  float2* d_frep = nullptr;         // [S][N0] forward values at the omit tilt
+  double* d_rowpart = nullptr;      // [S][N0] per-row partial sums
+  double* d_bnorm = nullptr;        // [S]
+  double* d_alpha = nullptr;        // [S]
+  int64_t device_bytes = 0;
+  // captured solve
+  hipGraphExec_t graph_exec = nullptr;
+  icon::GraphKey graph_key;
+};

@@ -1,0 +1,332 @@
+"""ctypes mirror of the MI355X ICON C ABI (include/icon_mi355x.h).
+
+The reference's in-process interface is ``Lib_2D2`` plus the ``cunfft_*`` plan
+API (tomography/ICON_GPU_v1.2.9/sourceCode/ICON-GPU/MainCode.cuh:10,
+nufft_gpu_v8.cuh:153-223).  This module exposes the same operations with the
+same argument meaning over the HIP library:
+
+* :class:`Plan` -- ``cunfft_initPlan_2d2`` + ``_1d`` + ``initGICONPara2`` for one
+  tilt series on one device, with batched forward / adjoint NUFFTs, the 1D
+  Fourier-slice transform of the projections and the ICON solve.
+* :func:`lib_2d2` -- ``Lib_2D2``: file-level reconstruction of a slice range.
+
+Arrays cross the boundary as device buffers (:class:`DeviceArray`) or as host
+numpy arrays that are copied in and out.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libicon_mi355x.so")
+_lib = None
+
+ICON_OK = 0
+_STATUS = {1: "ICON_ERR_ARG", 2: "ICON_ERR_DEVICE", 3: "ICON_ERR_IO", 4: "ICON_ERR_CAPACITY", 5: "ICON_ERR_STATE"}
+
+
+class IconError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (_STATUS.get(code, code), msg))
+        self.code = code
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("N0", ctypes.c_int32), ("N1", ctypes.c_int32), ("nz", ctypes.c_int32), ("M", ctypes.c_int32),
+                ("n0", ctypes.c_int32), ("n1", ctypes.c_int32), ("m", ctypes.c_int32), ("iomit", ctypes.c_int32),
+                ("sigma0", ctypes.c_float), ("sigma1", ctypes.c_float), ("b0", ctypes.c_float),
+                ("b1", ctypes.c_float), ("max_slices", ctypes.c_int32), ("col_block", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64)]
+
+
+class SolveOptions(ctypes.Structure):
+    """The reference's before[] / betain[] (ICONGPU.cu:380-387)."""
+    _fields_ = [("pre", ctypes.c_int32), ("icon", ctypes.c_int32), ("post", ctypes.c_int32),
+                ("data_type", ctypes.c_int32), ("threshold", ctypes.c_float), ("use_graph", ctypes.c_int32)]
+
+
+def lib_path():
+    return _LIB
+
+
+def library_loaded():
+    return _lib is not None
+
+
+def lib():
+    """Loads libicon_mi355x.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        raise ImportError("libicon_mi355x.so is not built; run __graft_entry__.build() "
+                          "(make -C feilab_amd/csrc)")
+    L = ctypes.CDLL(_LIB)
+    vp, ci, cf, cs = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+    L.icon_last_error.restype = ctypes.c_char_p
+    L.icon_version.restype = ctypes.c_char_p
+    L.icon_lib_2d2_error.restype = ctypes.c_char_p
+    sig = {
+        "icon_device_count": [ctypes.POINTER(ci)],
+        "icon_device_compute_units": [ci, ctypes.POINTER(ci)],
+        "icon_malloc": [ctypes.POINTER(vp), cs],
+        "icon_free": [vp],
+        "icon_memcpy_h2d": [vp, vp, cs],
+        "icon_memcpy_d2h": [vp, vp, cs],
+        "icon_memset_d": [vp, ci, cs],
+        "icon_device_synchronize": [],
+        "icon_plan_create": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
+        "icon_plan_destroy": [vp],
+        "icon_plan_get_info": [vp, ctypes.POINTER(PlanInfo)],
+        "icon_plan_set_col_block": [vp, ci],
+        "icon_nufft_forward_real": [vp, ci, vp, vp, vp],
+        "icon_nufft_forward_complex": [vp, ci, vp, vp, vp],
+        "icon_nufft_adjoint": [vp, ci, vp, vp, vp],
+        "icon_sino_to_b": [vp, ci, vp, vp, vp],
+        "icon_fft_rows": [ci, ci, ci, vp, vp],
+        "icon_fft_2d": [ci, ci, ci, ci, vp, vp],
+        "icon_reconstruct": [vp, ci, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolveOptions), vp],
+        "icon_reconstruct_host": [vp, ci, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolveOptions)],
+        "icon_lib_2d2": [ci, ci, ci, ci, ctypes.c_char_p, vp, ctypes.c_char_p, ctypes.c_char_p,
+                         ctypes.POINTER(ci), ctypes.POINTER(cf), ci],
+        "icon_read_tilts": [ctypes.c_char_p, ctypes.POINTER(cf), ci, ctypes.POINTER(ci)],
+        "icon_partition_slices": [ci, ctypes.POINTER(ci), ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ci
+    _lib = L
+    return L
+
+
+def _check(rc, what=""):
+    if rc != ICON_OK:
+        msg = lib().icon_last_error().decode(errors="replace")
+        raise IconError(rc, (what + ": " if what else "") + msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib().icon_device_count(ctypes.byref(n)), "icon_device_count")
+    return n.value
+
+
+class DeviceArray:
+    """A device buffer owned by Python (icon_malloc / icon_free)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        _check(lib().icon_malloc(ctypes.byref(self.ptr), max(self.nbytes, 1)), "icon_malloc")
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            _check(lib().icon_memcpy_h2d(d.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes), "icon_memcpy_h2d")
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().icon_memcpy_h2d(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes), "icon_memcpy_h2d")
+
+    def zero(self):
+        _check(lib().icon_memset_d(self.ptr, 0, self.nbytes), "icon_memset_d")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(lib().icon_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes),
+                   "icon_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr and self.ptr.value and _lib is not None:
+            _lib.icon_free(self.ptr)
+        self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return ctypes.c_void_p(int(x))
+
+
+class Plan:
+    """NUFFT/ICON plan of one tilt series on one device.
+
+    ``tilt_deg`` are the angles as written in the tilt file; the plan negates
+    them (MainCode.cu:42).  Images are [S][nx][thickness] (index X*N1 + Y),
+    node vectors [S][nz*nx] complex64, sinograms [S][nz][nx] float32.
+    """
+
+    def __init__(self, nx, tilt_deg, thickness=0, max_slices=1, device=0):
+        tilt = np.ascontiguousarray(tilt_deg, np.float32)
+        self.h = ctypes.c_void_p()
+        _check(lib().icon_plan_create(ctypes.byref(self.h), device, int(nx), int(thickness), len(tilt),
+                                      tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(max_slices)),
+               "icon_plan_create")
+        info = PlanInfo()
+        _check(lib().icon_plan_get_info(self.h, ctypes.byref(info)), "icon_plan_get_info")
+        self.info = info
+        self.N0, self.N1, self.nz, self.M = info.N0, info.N1, info.nz, info.M
+        self.n0, self.n1, self.iomit = info.n0, info.n1, info.iomit
+        self.max_slices = info.max_slices
+
+    def close(self):
+        if self.h and self.h.value and _lib is not None:
+            _lib.icon_plan_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_col_block(self, c):
+        _check(lib().icon_plan_set_col_block(self.h, int(c)), "icon_plan_set_col_block")
+        self.info.col_block = int(c)
+
+    # -- batched operators (device in / device out) -------------------------
+    def forward_real_dev(self, S, d_img, d_f, stream=None):
+        _check(lib().icon_nufft_forward_real(self.h, S, _ptr(d_img), _ptr(d_f), _ptr(stream)), "forward_real")
+
+    def forward_complex_dev(self, S, d_img, d_f, stream=None):
+        _check(lib().icon_nufft_forward_complex(self.h, S, _ptr(d_img), _ptr(d_f), _ptr(stream)), "forward_complex")
+
+    def adjoint_dev(self, S, d_f, d_img, stream=None):
+        _check(lib().icon_nufft_adjoint(self.h, S, _ptr(d_f), _ptr(d_img), _ptr(stream)), "adjoint")
+
+    def sino_to_b_dev(self, S, d_sino, d_b, stream=None):
+        _check(lib().icon_sino_to_b(self.h, S, _ptr(d_sino), _ptr(d_b), _ptr(stream)), "sino_to_b")
+
+    def reconstruct_dev(self, S, d_sino, d_m, opts, d_m_init=None, d_reproj=None, d_resid=None, d_alpha=None,
+                        stream=None):
+        _check(lib().icon_reconstruct(self.h, S, _ptr(d_sino), _ptr(d_m), _ptr(d_m_init), _ptr(d_reproj),
+                                      _ptr(d_resid), _ptr(d_alpha), ctypes.byref(opts), _ptr(stream)),
+               "icon_reconstruct")
+
+    # -- host conveniences ---------------------------------------------------
+    def forward(self, img):
+        img = np.asarray(img)
+        S = img.shape[0] if img.ndim == 3 else 1
+        cplx = np.iscomplexobj(img)
+        d_img = DeviceArray.from_host(img.reshape(S, self.N0, self.N1), np.complex64 if cplx else np.float32)
+        d_f = DeviceArray((S, self.M), np.complex64)
+        (self.forward_complex_dev if cplx else self.forward_real_dev)(S, d_img, d_f)
+        out = d_f.numpy()
+        return out if img.ndim == 3 else out[0]
+
+    def adjoint(self, f):
+        f = np.asarray(f)
+        S = f.shape[0] if f.ndim == 2 else 1
+        d_f = DeviceArray.from_host(f.reshape(S, self.M), np.complex64)
+        d_img = DeviceArray((S, self.N0, self.N1), np.complex64)
+        self.adjoint_dev(S, d_f, d_img)
+        out = d_img.numpy()
+        return out if f.ndim == 2 else out[0]
+
+    def sino_to_b(self, sino):
+        sino = np.asarray(sino, np.float32)
+        S = sino.shape[0] if sino.ndim == 3 else 1
+        d_s = DeviceArray.from_host(sino.reshape(S, self.nz, self.N0), np.float32)
+        d_b = DeviceArray((S, self.M), np.complex64)
+        self.sino_to_b_dev(S, d_s, d_b)
+        out = d_b.numpy()
+        return out if sino.ndim == 3 else out[0]
+
+    def solve(self, sino, pre, icon, post, data_type=2, threshold=0.0, m_init=None, trace=False,
+              reproject=False, use_graph=False):
+        """ICON solve of one or more slices (host arrays in and out)."""
+        sino = np.ascontiguousarray(sino, np.float32)
+        single = sino.ndim == 2
+        S = 1 if single else sino.shape[0]
+        iters = pre + icon + post
+        opts = SolveOptions(pre, icon, post, data_type, threshold, 1 if use_graph else 0)
+        m = np.zeros((S, self.N0, self.N1), np.float32)
+        mi = None if m_init is None else np.ascontiguousarray(m_init, np.float32).reshape(S, self.N0, self.N1)
+        rp = np.zeros((S, self.N0), np.float32) if reproject else None
+        rs = np.zeros((S, iters + 1)) if trace else None
+        al = np.zeros((S, max(iters, 1))) if trace else None
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+        _check(lib().icon_reconstruct_host(self.h, S, P(sino), P(m), P(mi), P(rp), P(rs), P(al),
+                                           ctypes.byref(opts)), "icon_reconstruct_host")
+        out = [m[0] if single else m]
+        if reproject:
+            out.append(rp[0] if single else rp)
+        if trace:
+            out.append(rs[0] if single else rs)
+            out.append((al[0] if single else al)[..., :iters])
+        return out[0] if len(out) == 1 else tuple(out)
+
+
+def fft_rows(x, sign=-1):
+    x = np.ascontiguousarray(x, np.complex64)
+    n = x.shape[-1]
+    d = DeviceArray.from_host(x)
+    _check(lib().icon_fft_rows(n, x.size // n, sign, d.ptr, None), "icon_fft_rows")
+    return d.numpy()
+
+
+def fft_2d(x, sign=-1):
+    x = np.ascontiguousarray(x, np.complex64)
+    n0, n1 = x.shape[-2:]
+    d = DeviceArray.from_host(x)
+    _check(lib().icon_fft_2d(n0, n1, x.size // (n0 * n1), sign, d.ptr, None), "icon_fft_2d")
+    return d.numpy()
+
+
+def read_tilts(path, cap=1000):
+    buf = (ctypes.c_float * cap)()
+    n = ctypes.c_int(0)
+    rc = lib().icon_read_tilts(path.encode(), buf, cap, ctypes.byref(n))
+    if rc == 3:
+        raise IconError(rc, "Can not open anglefile '%s'!" % path)
+    if rc:
+        raise IconError(rc, "tilt file holds more than %d values" % cap)
+    return np.array(buf[:n.value], np.float32)
+
+
+def partition_slices(cus, y_start, y_end):
+    cus = list(cus)
+    nd = len(cus)
+    c = (ctypes.c_int * nd)(*cus)
+    f = (ctypes.c_int * nd)()
+    l_ = (ctypes.c_int * nd)()
+    used = lib().icon_partition_slices(nd, c, y_start, y_end, f, l_)
+    if used < 0 or used > nd:
+        raise IconError(1, "partition failed")
+    return [(f[i], l_[i]) for i in range(used)]
+
+
+def lib_2d2(mrc_path, angle_path, result_dir, slice_begin, slice_end, pre, icon, post, data_type=1,
+            threshold=0.03, thickness=0, device=0, batch=0):
+    """Lib_2D2 (MainCode.cu:3-177): reconstruct [slice_begin, slice_end) of a tilt series into
+    result_dir/midYYYYY.mrc; returns the reprojection rows [slices][nx]."""
+    import struct
+    with open(mrc_path, "rb") as fh:
+        nx = struct.unpack("<i", fh.read(4))[0]
+    rp = np.zeros((max(slice_end - slice_begin, 0), nx), np.float32)
+    before = (ctypes.c_int * 5)(pre, post, 1, data_type, 1)
+    betain = (ctypes.c_float * 2)(threshold, float(thickness))
+    rc = lib().icon_lib_2d2(device, slice_begin, slice_end, pre + icon + post, mrc_path.encode(),
+                            rp.ctypes.data_as(ctypes.c_void_p), angle_path.encode(), result_dir.encode(), before,
+                            betain, batch)
+    if rc:
+        raise IconError(rc, lib().icon_lib_2d2_error().decode(errors="replace"))
+    return rp

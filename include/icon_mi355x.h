@@ -1,0 +1,165 @@
+/* icon_mi355x.h -- C ABI of the MI355X ICON reconstruction core.
+ *
+ * Drop-in boundary for ICON-GPU's per-slice compressed-sensing NUFFT solve.
+ * Plain C types only: device buffers are `void*` obtained from icon_malloc()
+ * (or any hipMalloc'd pointer on the plan's device), streams are `void*`
+ * (a hipStream_t, NULL = the default stream).  Every call returns an
+ * icon_status; icon_last_error() describes the last failure on the calling
+ * thread.  Nothing here calls exit().
+ *
+ * Reference interfaces replaced (paths relative to
+ * tomography/ICON_GPU_v1.2.9/sourceCode/ICON-GPU/):
+ *   Lib_2D2                 MainCode.cuh:10, MainCode.cu:3-177
+ *   Iteration3 / Alpha2     Iteration2.cuh:3, alpha_new2.cu:3-119
+ *   cunfft_*Plan_2d2/_1d    nufft_gpu_v8.cuh:153-223, nufft_gpu_v8.cu:281-556,1298-1408
+ *   cunfft_trafo_2d_R2C2    nufft_gpu_v8.cu:984-1076
+ *   cunfft_trafo_2d2        nufft_gpu_v8.cu:844-949
+ *   cunfft_adjoint_2d2      nufft_gpu_v8.cu:1167-1238
+ *   cunfft_trafo_1d         nufft_gpu_v8.cu:1479-1516 (readSlicePart.cu:10-22)
+ *   cunfft_adjoint_1d       nufft_gpu_v8.cu:1518-1536 (reProject_NFFT.cu:81-98)
+ *   initGICONPara2          nufft_gpu_v8.cu:1739-1771
+ */
+#ifndef ICON_MI355X_H
+#define ICON_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  ICON_OK = 0,
+  ICON_ERR_ARG = 1,      /* invalid argument (sizes, pointers, flags) */
+  ICON_ERR_DEVICE = 2,   /* HIP runtime failure */
+  ICON_ERR_IO = 3,       /* file could not be opened / read / written */
+  ICON_ERR_CAPACITY = 4, /* batch larger than the plan was created for */
+  ICON_ERR_STATE = 5     /* call out of order (e.g. plan not prepared) */
+} icon_status;
+
+typedef struct icon_plan icon_plan;
+
+/* Geometry of a plan, as derived by the reference's setup
+ * (MainCode.cu:63-71, nufft_gpu_v8.cu:471-476). */
+typedef struct {
+  int32_t N0, N1;        /* width (nx) and thickness */
+  int32_t nz;            /* number of tilts */
+  int32_t M;             /* nodes = N0 * nz */
+  int32_t n0, n1;        /* oversampled grid, n = 2 * nextpow2(N) */
+  int32_t m;             /* window half-width, 6 (14 taps) */
+  int32_t iomit;         /* first tilt of minimal |theta| (MainCode.cu:46-49) */
+  float sigma0, sigma1;  /* oversampling factors, float as the reference */
+  float b0, b1;          /* Gaussian window shape parameters */
+  int32_t max_slices;    /* batch capacity */
+  int32_t col_block;     /* grid columns per spread/interp tile */
+  int64_t device_bytes;  /* device memory held by the plan */
+} icon_plan_info;
+
+/* Solver parameters: the reference's before[] / betain[] (ICONGPU.cu:380-387). */
+typedef struct {
+  int32_t pre;        /* INFR iterations before ICON            (-iter a,_,_) */
+  int32_t icon;       /* ICON (thresholded) iterations          (-iter _,b,_) */
+  int32_t post;       /* INFR iterations after ICON             (-iter _,_,c) */
+  int32_t data_type;  /* 1: clamp m <= thr, 2: clamp m >= thr   (-d)        */
+  float threshold;    /*                                        (-thr)      */
+  int32_t use_graph;  /* 1: replay the iteration loop as a HIP graph       */
+} icon_solve_opts;
+
+/* ---- errors / devices / memory ---------------------------------------- */
+const char* icon_last_error(void);
+const char* icon_version(void);
+int icon_device_count(int* count);
+/* Device memory helpers on the current device (thin HIP wrappers). */
+int icon_malloc(void** dptr, size_t bytes);
+int icon_free(void* dptr);
+int icon_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int icon_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int icon_memset_d(void* dst, int value, size_t bytes);
+int icon_device_synchronize(void);
+
+/* ---- plans ------------------------------------------------------------- */
+/* Create a plan on `device` for a tilt series of width nx, reconstruction
+ * thickness (0 = nx), nz tilts at the angles of the tilt file (degrees, as
+ * read: the plan negates them as MainCode.cu:42 does) and up to max_slices
+ * slices per batch.  Replaces cunfft_mallocPlan/initPlan_2d2 + _1d +
+ * initGICONPara2. */
+int icon_plan_create(icon_plan** plan, int device, int nx, int thickness, int nz,
+                     const float* tilt_deg, int max_slices);
+int icon_plan_destroy(icon_plan* plan);
+int icon_plan_get_info(const icon_plan* plan, icon_plan_info* info);
+/* Tune the spread/interp column tile (8 or 16 grid columns); rebuilds the
+ * bin lists.  Optional. */
+int icon_plan_set_col_block(icon_plan* plan, int col_block);
+
+/* ---- operators on S slices (device pointers, batched) ------------------ */
+/* Forward NUFFT of S real images [S][N0][N1] to S node vectors [S][M]
+ * complex (interleaved float2).  cunfft_trafo_2d_R2C2. */
+int icon_nufft_forward_real(icon_plan* plan, int S, const void* d_img, void* d_f, void* stream);
+/* Forward NUFFT of complex images [S][N0][N1] (float2).  cunfft_trafo_2d2. */
+int icon_nufft_forward_complex(icon_plan* plan, int S, const void* d_img, void* d_f, void* stream);
+/* Adjoint NUFFT of node vectors [S][M] (float2) to images [S][N0][N1]
+ * (float2).  cunfft_adjoint_2d2. */
+int icon_nufft_adjoint(icon_plan* plan, int S, const void* d_f, void* d_img, void* stream);
+/* Fourier-slice data of S sinograms [S][nz][N0] (float): the 1D NUFFT of
+ * every projection row, [S][M] float2.  readSlicePart + cunfft_trafo_1d. */
+int icon_sino_to_b(icon_plan* plan, int S, const void* d_sino, void* d_b, void* stream);
+/* Batched 1D FFT of `rows` contiguous rows of length n (256..2048, power of
+ * two), in place, sign -1 (forward) or +1 (inverse), unnormalised. */
+int icon_fft_rows(int n, int rows, int sign, void* d_data, void* stream);
+/* Batched 2D FFT of `count` n0 x n1 grids (row-major, index x*n1 + y). */
+int icon_fft_2d(int n0, int n1, int count, int sign, void* d_data, void* stream);
+
+/* ---- the solve ---------------------------------------------------------- */
+/* ICON/INFR solve of S slices (MainCode.cu:113-162, Iteration2.cu:3-150):
+ *   d_sino    [S][nz][N0] float  projection rows of each slice (input)
+ *   d_m       [S][N0][N1] float  reconstruction, index X*N1+Y (output)
+ *   d_m_init  [S][N0][N1] float  warm start, or NULL for zeros (reference)
+ *   d_reproj  [S][N0]     float  reprojection at the min-|theta| tilt, or NULL
+ *   d_resid   [S][iters+1] double data residual R(m_i), i = 0..iters, or NULL:
+ *             R(m) = sqrt(sum_j w_j |(A m)_j - b_j|^2 / sum_j w_j |b_j|^2)
+ *   d_alpha   [S][iters]  double step sizes, or NULL
+ * All pointers are device pointers on the plan's device; the call is
+ * asynchronous on `stream`. */
+int icon_reconstruct(icon_plan* plan, int S, const void* d_sino, void* d_m, const void* d_m_init,
+                     void* d_reproj, void* d_resid, void* d_alpha, const icon_solve_opts* opts,
+                     void* stream);
+
+/* Host-buffer convenience wrapper of icon_reconstruct (synchronous). */
+int icon_reconstruct_host(icon_plan* plan, int S, const float* sino, float* m, const float* m_init,
+                          float* reproj, double* resid, double* alpha, const icon_solve_opts* opts);
+
+/* ---- file-level entry point -------------------------------------------- */
+/* Lib_2D2 (MainCode.cu:3-177): reconstruct slices [slice_begin, slice_end)
+ * of the MRC tilt series `mrc_path` with the tilt file `angle_path`, writing
+ * <result_dir>/midYYYYY.mrc per slice (f32, nx x thickness, transposed as
+ * saveMRC_real_inv) and the reprojection rows into `reproj`
+ * ((slice_end - slice_begin) * nx floats, may be NULL).  `before` =
+ * {pre, post, threadnum, dataType, skipCV}, `betain` = {threshold,
+ * thickness}; iternum = pre + icon + post.  Slices are solved in batches of
+ * up to `batch` (<= 0: automatic) on `device`. */
+int icon_lib_2d2(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
+                 float* reproj, const char* angle_path, const char* result_dir, const int* before,
+                 const float* betain, int batch);
+/* Last failure of icon_lib_2d2 on any thread. */
+const char* icon_lib_2d2_error(void);
+
+/* Tilt file reader (MainCode.cu:38-45): whitespace-separated degrees, as
+ * written (not negated).  Stores at most `cap`, reports the count found;
+ * ICON_ERR_IO if the file cannot be opened, ICON_ERR_CAPACITY if it holds
+ * more than `cap` values. */
+int icon_read_tilts(const char* path, float* tilt_deg, int cap, int* count);
+
+/* Slice partition across devices (ICONGPU.cu:393-431): shares proportional
+ * to the compute-unit counts `cus`, leftovers round-robin from the device of
+ * most units, the last device taking the remainder.  Fills first[d]/last[d]
+ * (inclusive) and returns the number of devices used. */
+int icon_partition_slices(int ndev, const int* cus, int y_start, int y_end, int* first, int* last);
+
+/* Compute units of a device (the partition weight). */
+int icon_device_compute_units(int device, int* cus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICON_MI355X_H */

@@ -1,0 +1,186 @@
+"""GPU parity of the HIP path against the oracle (FP32 mirror and FP64).
+
+Every test calls the library through the C ABI (ctypes) and compares with the
+CPU restatement of the reference on the same seeded inputs.  Tolerances:
+
+* FFT kernels vs numpy (FP64): relative L2 <= 2e-6 (FP32 radix-16 Stockham).
+* NUFFT forward / adjoint / 1D transform vs the FP32-mirror oracle: relative
+  L2 <= 2e-5 (FFT rounding order differs from the oracle's radix-2).
+* Whole ICON solves vs the FP32-mirror oracle: relative L2 <= 1e-4 per slice
+  (the bar for "the reference's own reconstruction"), data residual within
+  1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from feilab_amd import phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_rows_vs_numpy(icon, n, sign):
+    _print("fft_rows", n, sign)
+    rng = np.random.default_rng(n + sign)
+    x = (rng.standard_normal((37, n)) + 1j * rng.standard_normal((37, n))).astype(np.complex64)
+    y = icon.fft_rows(x, sign)
+    ref = np.fft.fft(x.astype(np.complex128), axis=-1) if sign < 0 else np.fft.ifft(x.astype(np.complex128), axis=-1) * n
+    assert rel(y, ref) < 2e-6
+
+
+@pytest.mark.parametrize("shape", [(256, 256), (512, 1024), (1024, 1024), (2048, 512)])
+def test_fft_2d_vs_numpy(icon, shape):
+    _print("fft_2d", shape)
+    rng = np.random.default_rng(shape[0])
+    x = (rng.standard_normal((2,) + shape) + 1j * rng.standard_normal((2,) + shape)).astype(np.complex64)
+    y = icon.fft_2d(x, -1)
+    ref = np.fft.fft2(x.astype(np.complex128))
+    assert rel(y, ref) < 3e-6
+    z = icon.fft_2d(x, 1)
+    assert rel(z, np.fft.ifft2(x.astype(np.complex128)) * shape[0] * shape[1]) < 3e-6
+
+
+GEOMS = [
+    (64, 64, phantom.tilt_series(60, 6)),      # small, sigma = 2
+    (100, 72, phantom.tilt_series(50, 5)),     # non-power-of-two width and thickness
+    (256, 256, phantom.tilt_series(60, 2)),    # config 1 shape
+]
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "%dx%d_nz%d" % (g[0], g[1], len(g[2])))
+def test_nufft_forward_adjoint_vs_oracle(icon, oracle_mod, geom):
+    N0, N1, tilt = geom
+    _print("nufft", N0, N1, len(tilt))
+    rng = np.random.default_rng(N0 * 7 + N1)
+    plan = icon.Plan(N0, tilt, thickness=N1, max_slices=3)
+    orc = oracle_mod.Plan(N0, N1, tilt, 32)
+    assert (plan.n0, plan.n1, plan.M, plan.iomit) == (orc.n0, orc.n1, orc.M, orc.iomit)
+    img = rng.standard_normal((3, N0, N1)).astype(np.float32)
+    f = plan.forward(img)
+    for s in range(3):
+        assert rel(f[s], orc.forward(img[s].astype(np.complex128))) < 2e-5
+    cimg = (rng.standard_normal((2, N0, N1)) + 1j * rng.standard_normal((2, N0, N1))).astype(np.complex64)
+    fc = plan.forward(cimg)
+    for s in range(2):
+        assert rel(fc[s], orc.forward(cimg[s])) < 2e-5
+    g = (rng.standard_normal((3, plan.M)) + 1j * rng.standard_normal((3, plan.M))).astype(np.complex64)
+    a = plan.adjoint(g)
+    for s in range(3):
+        assert rel(a[s], orc.adjoint(g[s])) < 2e-5
+    # adjointness of the device operators themselves
+    lhs = np.vdot(g[0].astype(np.complex128), fc[0])
+    rhs = np.vdot(a[0].astype(np.complex128), cimg[0])
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+@pytest.mark.parametrize("geom", GEOMS[:2], ids=lambda g: "%dx%d" % (g[0], g[1]))
+def test_nufft_vs_fp64_ndft(icon, oracle_mod, geom):
+    """The device NUFFT against the exact transform: the Gaussian-window error
+    bound at m = 6, sigma >= 2 is 4 exp(-m pi (1 - 1/(2 sigma - 1))) ~ 1.4e-5."""
+    N0, N1, tilt = geom
+    rng = np.random.default_rng(3)
+    plan = icon.Plan(N0, tilt, thickness=N1)
+    orc = oracle_mod.Plan(N0, N1, tilt, 64)
+    nodes = orc.tables()["nodes"]
+    img = rng.standard_normal((N0, N1))
+    exact = oracle_mod.ndft_forward(nodes, img.astype(np.complex128))
+    assert rel(plan.forward(img.astype(np.float32)), exact) < 3e-5
+
+
+def test_sino_to_b_vs_oracle(icon, oracle_mod):
+    cfg = phantom.config1()
+    plan = icon.Plan(256, cfg["tilt"], max_slices=2)
+    orc = oracle_mod.Plan(256, 256, cfg["tilt"], 32)
+    sinos = np.stack([cfg["sino"], cfg["clean"]])
+    b = plan.sino_to_b(sinos)
+    for s in range(2):
+        assert rel(b[s], orc.sino_to_b(sinos[s])) < 1e-5
+
+
+def _small_case(seed, N=96, nb=12, step=3):
+    rng = np.random.default_rng(seed)
+    tilt = phantom.tilt_series(60, step)
+    b = phantom.blobs_2d(N, nb, rng)
+    clean = phantom.project_2d(b, N, tilt)
+    return tilt, phantom.add_noise(clean, 0.5, rng)
+
+
+@pytest.mark.parametrize("data_type,thr", [(2, 0.0), (1, 0.03)])
+def test_icon_solve_small_vs_oracle(icon, oracle_mod, data_type, thr):
+    _print("solve small", data_type, thr)
+    tilt, sino = _small_case(11)
+    N = sino.shape[1]
+    plan = icon.Plan(N, tilt, max_slices=1)
+    orc = oracle_mod.Plan(N, N, tilt, 32)
+    m, rp, res, al = plan.solve(sino, 5, 30, 5, data_type, thr, trace=True, reproject=True)
+    mo, reso, alo = orc.solve(sino, 5, 30, 5, data_type, thr, trace=True)
+    assert rel(m, mo) < 1e-4
+    assert np.allclose(res, reso, rtol=1e-4, atol=1e-7)
+    assert np.allclose(al, alo, rtol=1e-4)
+    assert rel(rp, orc.reproject(mo)) < 1e-4
+
+
+def test_icon_solve_config1_vs_oracle(icon, oracle_mod):
+    """Config 1 (256^2, 61 tilts, 100 iterations) against the FP32 mirror."""
+    cfg = phantom.config1()
+    _print("solve config1")
+    plan = icon.Plan(256, cfg["tilt"], max_slices=1)
+    orc = oracle_mod.Plan(256, 256, cfg["tilt"], 32)
+    m, res, al = plan.solve(cfg["sino"], cfg["pre"], cfg["icon"], cfg["post"], 2, 0.0, trace=True)
+    mo, reso, alo = orc.solve(cfg["sino"], cfg["pre"], cfg["icon"], cfg["post"], 2, 0.0, trace=True)
+    _print("config1 rel", rel(m, mo), "resid", res[-1], reso[-1])
+    assert rel(m, mo) < 1e-4
+    assert abs(res[-1] - reso[-1]) / reso[-1] < 1e-4
+
+
+def test_batch_matches_single(icon):
+    """Slices of a batch are independent: solving them together gives each
+    slice's single-slice result."""
+    tilt, s0 = _small_case(21)
+    _, s1 = _small_case(22)
+    N = s0.shape[1]
+    plan = icon.Plan(N, tilt, max_slices=2)
+    mb = plan.solve(np.stack([s0, s1]), 3, 10, 3, 2, 0.0)
+    m0 = plan.solve(s0, 3, 10, 3, 2, 0.0)
+    m1 = plan.solve(s1, 3, 10, 3, 2, 0.0)
+    assert rel(mb[0], m0) < 1e-6 and rel(mb[1], m1) < 1e-6
+
+
+def test_graph_replay_matches_eager(icon):
+    """The captured solve, replayed several times with eager work in between,
+    gives the eager result every time."""
+    tilt, s0 = _small_case(31)
+    N = s0.shape[1]
+    plan = icon.Plan(N, tilt, max_slices=1)
+    eager = plan.solve(s0, 3, 10, 3, 2, 0.0, use_graph=False)
+    from feilab_amd.icon import DeviceArray, SolveOptions
+    d_s = DeviceArray.from_host(s0[None])
+    d_m = DeviceArray((1, N, N), np.float32)
+    opts = SolveOptions(3, 10, 3, 2, 0.0, 1)
+    for rep in range(3):
+        d_m.zero()
+        plan.reconstruct_dev(1, d_s, d_m, opts)
+        icon.lib().icon_device_synchronize()
+        assert rel(d_m.numpy()[0], eager) < 1e-6, rep
+        plan.solve(s0, 1, 1, 1, 2, 0.0)  # eager work between replays
+
+
+def test_warm_start_continues(icon, oracle_mod):
+    """A warm start m_init replaces the zero start (Iteration2.cu:33)."""
+    tilt, s0 = _small_case(41)
+    N = s0.shape[1]
+    plan = icon.Plan(N, tilt)
+    orc = oracle_mod.Plan(N, N, tilt, 32)
+    m_init = orc.solve(s0, 2, 5, 0, 2, 0.0)
+    m = plan.solve(s0, 0, 10, 2, 2, 0.0, m_init=m_init)
+    mo = orc.solve(s0, 0, 10, 2, 2, 0.0, m_init=m_init)
+    assert rel(m, mo) < 1e-4
