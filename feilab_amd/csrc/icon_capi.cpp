@@ -88,7 +88,13 @@ void free_bins(icon_plan* p) {
   if (p->d_ent_off) hipFree(p->d_ent_off);
   if (p->d_ent) hipFree(p->d_ent);
   if (p->d_nslot) hipFree(p->d_nslot);
-  p->d_ent_off = nullptr; p->d_ent = nullptr; p->d_nslot = nullptr;
+  if (p->d_xoff) hipFree(p->d_xoff);
+  if (p->d_ent_px) hipFree(p->d_ent_px);
+  if (p->d_ent_py) hipFree(p->d_ent_py);
+  if (p->d_ent_of_node) hipFree(p->d_ent_of_node);
+  if (p->d_fw) hipFree(p->d_fw);
+  p->d_ent_off = nullptr; p->d_ent = nullptr; p->d_nslot = nullptr; p->d_xoff = nullptr;
+  p->d_ent_px = nullptr; p->d_ent_py = nullptr; p->d_ent_of_node = nullptr; p->d_fw = nullptr;
 }
 
 int upload_bins(icon_plan* p) {
@@ -96,6 +102,12 @@ int upload_bins(icon_plan* p) {
   if ((rc = upload(p, &p->d_ent_off, p->bins.off))) return rc;
   if ((rc = upload(p, &p->d_ent, p->bins.ent))) return rc;
   if ((rc = upload(p, &p->d_nslot, p->bins.nslot))) return rc;
+  if ((rc = upload(p, &p->d_xoff, p->bins.xoff))) return rc;
+  if ((rc = upload(p, &p->d_ent_px, p->bins.px))) return rc;
+  if ((rc = upload(p, &p->d_ent_py, p->bins.py))) return rc;
+  if ((rc = upload(p, &p->d_ent_of_node, p->bins.of_node))) return rc;
+  // weighted node value per bin entry, per slice
+  if ((rc = dalloc(p, &p->d_fw, (size_t)p->S_max * p->bins.ent.size()))) return rc;
   return ICON_OK;
 }
 
@@ -105,7 +117,9 @@ void drop_graph(icon_plan* p) {
   p->graph_key = GraphKey{};
 }
 
-int default_col_block(int n0) { return n0 >= 2048 ? 8 : 16; }
+// Eight grid columns per tile: two blocks fit in one CU's LDS at n0 = 1024
+// (measured faster than 16 columns at one block per CU).
+int default_col_block(int n0) { (void)n0; return 8; }
 
 struct DevGuard {
   int prev = -1;
@@ -136,7 +150,7 @@ int op_forward(icon_plan* p, int S, const void* img, int cplx, float2* out, hipS
   if (rc) return rc;
   ColArgs ca{};
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
-  ca.in = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent;
+  ca.in = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.part = p->d_part;
   if ((rc = check_launch(launch_col(g.n0, p->bins.C, false, ca, st), "col_fwd"))) return rc;
   CombineArgs cb{};
@@ -145,11 +159,13 @@ int op_forward(icon_plan* p, int S, const void* img, int cplx, float2* out, hipS
   return check_launch(launch_combine(cb, st), "combine");
 }
 
-int op_adjoint(icon_plan* p, int S, const float2* f, float2* img, hipStream_t st) {
+// A^H of the per-entry weighted values already in p->d_fw.
+int op_adjoint_ent(icon_plan* p, int S, float2* img, hipStream_t st) {
   const Geometry& g = p->g;
+  const float2* f = p->d_fw;
   ColArgs ca{};
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
-  ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent;
+  ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.fw = f;
   int rc = check_launch(launch_col(g.n0, p->bins.C, true, ca, st), "col_adj");
   if (rc) return rc;
@@ -157,6 +173,14 @@ int op_adjoint(icon_plan* p, int S, const float2* f, float2* img, hipStream_t st
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 0; ra.in = p->d_tmp;
   ra.ck = p->d_ck; ra.tw = p->d_tw1; ra.img = img;
   return check_launch(launch_row_adj(g.n1, ra, st), "row_adj");
+}
+
+// A^H of node vectors f [S][M].
+int op_adjoint(icon_plan* p, int S, const float2* f, float2* img, hipStream_t st) {
+  int rc = check_launch(launch_node_to_ent(f, p->d_fw, p->d_ent, S, p->g.M, (int)p->bins.ent.size(), st),
+                        "node_to_ent");
+  if (rc) return rc;
+  return op_adjoint_ent(p, S, img, st);
 }
 
 int op_sino(icon_plan* p, int S, const float* sino, float2* b, hipStream_t st) {
@@ -179,9 +203,10 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   CombineArgs c0{};
   c0.S = S; c0.M = g.M; c0.N0 = g.N0; c0.nz = g.nz; c0.mode = 2; c0.b = p->d_b; c0.w = p->d_w;
   c0.fw = p->d_fw; c0.bnorm = p->d_bnorm; c0.nslot = p->d_nslot; c0.part = p->d_part;
+  c0.E = (int)p->bins.ent.size(); c0.ent_of_node = p->d_ent_of_node;
   if ((rc = check_launch(launch_combine(c0, st), "combine_b"))) return rc;
   // A^H W b
-  if ((rc = op_adjoint(p, S, p->d_fw, p->d_ahwb, st))) return rc;
+  if ((rc = op_adjoint_ent(p, S, p->d_ahwb, st))) return rc;
   // m = 0 (Iteration2.cu:33) or the warm start
   const size_t NN = (size_t)g.N0 * g.N1;
   if (m_init) rc = check_launch(launch_copy(m, m_init, (size_t)S * NN, st), "copy");
@@ -193,12 +218,13 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   rf.m = m; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha; rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
   ColArgs ca{};
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
-  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
+  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
   CombineArgs cm{};
   cm.S = S; cm.M = g.M; cm.N0 = g.N0; cm.nz = g.nz; cm.mode = 0; cm.part = p->d_part; cm.nslot = p->d_nslot;
   cm.w = p->d_w; cm.b = p->d_b; cm.fw = p->d_fw; cm.bnorm = p->d_bnorm; cm.resid = resid;
   cm.resid_stride = iters + 1; cm.iomit = g.iomit;
+  cm.E = (int)p->bins.ent.size(); cm.ent_of_node = p->d_ent_of_node;
   CombineArgs cr = cm;
   cr.mode = 1; cr.resid = nullptr; cr.rowpart = p->d_rowpart; cr.alpha = p->d_alpha; cr.alpha_trace = alpha_tr;
   cr.alpha_stride = iters;
@@ -311,6 +337,28 @@ int icon_device_synchronize(void) {
   return ICON_OK;
 }
 
+int icon_geometry_tables(int nx, int thickness, int nz, const float* tilt_deg, icon_plan_info* info,
+                         float* nodes, float* weights, float* phix, float* phiy, int* ix0, int* iy0, float* ck,
+                         float* ck1, float* phi1, int* i10) {
+  if (!tilt_deg) return fail(ICON_ERR_ARG, "icon_geometry_tables: null tilt angles");
+  Geometry g;
+  std::string err;
+  if (!build_geometry(g, nx, thickness > 0 ? thickness : nx, nz, tilt_deg, &err))
+    return fail(ICON_ERR_ARG, "%s", err.c_str());
+  auto put = [](auto* dst, const auto& v) {
+    if (dst) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
+  };
+  put(nodes, g.nodes); put(weights, g.w); put(phix, g.phix); put(phiy, g.phiy); put(ix0, g.ix0);
+  put(iy0, g.iy0); put(ck, g.ck); put(ck1, g.ck1); put(phi1, g.phi1); put(i10, g.i10);
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    info->N0 = g.N0; info->N1 = g.N1; info->nz = g.nz; info->M = g.M; info->n0 = g.n0; info->n1 = g.n1;
+    info->m = kWindowM; info->iomit = g.iomit; info->sigma0 = g.sig0; info->sigma1 = g.sig1;
+    info->b0 = g.b0; info->b1 = g.b1;
+  }
+  return ICON_OK;
+}
+
 int icon_plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg,
                      int max_slices) {
   if (!out || !tilt_deg) return fail(ICON_ERR_ARG, "icon_plan_create: null pointer");
@@ -349,7 +397,6 @@ int icon_plan_create(icon_plan** out, int device, int nx, int thickness, int nz,
   if ((rc = twiddles(g.n1, &p->d_tw1))) return bail(rc);
   const size_t S = (size_t)max_slices, M = (size_t)g.M, NN = (size_t)g.N0 * g.N1;
   if ((rc = dalloc(p, &p->d_b, S * M))) return bail(rc);
-  if ((rc = dalloc(p, &p->d_fw, S * M))) return bail(rc);
   if ((rc = dalloc(p, &p->d_part, S * M * kMaxSlot))) return bail(rc);
   if ((rc = dalloc(p, &p->d_ahwb, S * NN))) return bail(rc);
   if ((rc = dalloc(p, &p->d_rhat, S * NN))) return bail(rc);
@@ -380,7 +427,7 @@ int icon_plan_destroy(icon_plan* p) {
   if (p->stream) hipStreamSynchronize(p->stream);
   drop_graph(p);
   void* ptrs[] = {p->d_ck, p->d_ck1, p->d_w, p->d_phix, p->d_phiy, p->d_ix0, p->d_phi1, p->d_i10,
-                  p->d_b, p->d_fw, p->d_part, p->d_ahwb, p->d_rhat, p->d_tmp, p->d_frep, p->d_rowpart,
+                  p->d_b, p->d_part, p->d_ahwb, p->d_rhat, p->d_tmp, p->d_frep, p->d_rowpart,
                   p->d_bnorm, p->d_alpha};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -558,6 +605,77 @@ int icon_reconstruct_host(icon_plan* p, int S, const float* sino, float* m, cons
   if (alpha && iters > 0) TRY(hipMemcpy(alpha, dal, (size_t)S * iters * sizeof(double), hipMemcpyDeviceToHost));
 #undef TRY
   cleanup();
+  return ICON_OK;
+}
+
+int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) {
+  if (!p || !avg_ms || reps < 1) return fail(ICON_ERR_ARG, "icon_time_kernel: bad argument");
+  if (S < 1 || S > p->S_max) return fail(ICON_ERR_CAPACITY, "batch %d outside 1..%d", S, p->S_max);
+  DevGuard dg(p->device);
+  const Geometry& g = p->g;
+  hipStream_t st = p->stream;
+  RowFwdArgs rf{};
+  rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.cplx = 1; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
+  rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
+  ColArgs ca{};
+  ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
+  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
+  ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
+  CombineArgs cm{};
+  cm.S = S; cm.M = g.M; cm.N0 = g.N0; cm.nz = g.nz; cm.mode = 0; cm.part = p->d_part; cm.nslot = p->d_nslot;
+  cm.w = p->d_w; cm.b = p->d_b; cm.fw = p->d_fw; cm.bnorm = p->d_bnorm;
+  cm.E = (int)p->bins.ent.size(); cm.ent_of_node = p->d_ent_of_node;
+  RowAdjArgs ra{};
+  ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 1; ra.in = p->d_tmp; ra.ck = p->d_ck;
+  ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart;
+  SinoArgs sa{};
+  sa.rows = S * g.nz; sa.N0 = g.N0; sa.sino = reinterpret_cast<const float*>(p->d_b); sa.ck1 = p->d_ck1;
+  sa.phi1 = p->d_phi1; sa.i10 = p->d_i10; sa.tw = p->d_tw0; sa.b = p->d_fw;
+  auto launch = [&]() -> hipError_t {
+    switch (kernel) {
+      case ICON_K_ROW_FWD: return launch_row_fwd(g.n1, rf, st);
+      case ICON_K_COL_FWD: return launch_col(g.n0, p->bins.C, false, ca, st);
+      case ICON_K_COMBINE: return launch_combine(cm, st);
+      case ICON_K_COL_ADJ: return launch_col(g.n0, p->bins.C, true, ca, st);
+      case ICON_K_ROW_ADJ: return launch_row_adj(g.n1, ra, st);
+      case ICON_K_SINO: return launch_sino(g.n0, sa, st);
+      default: return hipErrorInvalidValue;
+    }
+  };
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipError_t e = launch();  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "icon_time_kernel: %s", hipGetErrorString(e));
+  *avg_ms = (double)ms / reps;
+  return ICON_OK;
+}
+
+int icon_copy_probe(void* d_dst, const void* d_src, size_t bytes, int reps, double* avg_ms) {
+  if (!d_dst || !d_src || !avg_ms || reps < 1 || bytes % 16 != 0)
+    return fail(ICON_ERR_ARG, "icon_copy_probe: bad argument (bytes must be a multiple of 16)");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipError_t e = launch_copy4((float4*)d_dst, (const float4*)d_src, bytes / 16, nullptr);
+  if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy4((float4*)d_dst, (const float4*)d_src, bytes / 16, nullptr);
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "icon_copy_probe: %s", hipGetErrorString(e));
+  *avg_ms = (double)ms / reps;
   return ICON_OK;
 }
 

@@ -13,6 +13,7 @@
 //   1D nodes              MainCode.cu:75-78
 //   density weights       weight.cu:8-41
 //   omitted angle         MainCode.cu:46-49
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -175,15 +176,16 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
 // Bins every node's run of y-taps by grid-column tile.  A node's 14 taps are
 // consecutive columns (mod n1), so they fall into at most ceil(13/C)+1 tiles;
 // each (tile, node) pair is one entry recording the tap range, the node's
-// partial-sum slot and the first column inside the tile.  Entries of a tile
-// are in ascending node order.
+// partial-sum slot, the first column inside the tile and the node's first
+// x-tap.  A tile's entries are sorted by first x-tap (then node), and xoff
+// gives, per tile and grid row x, the first entry whose first x-tap is >= x,
+// so the entries covering row x are those with first x-tap in [x-13, x].
 void build_bins(const Geometry& g, int C, Bins& b) {
   b.C = C;
   b.nblk = g.n1 / C;
   b.off.assign(b.nblk + 1, 0);
   b.nslot.assign(g.M, 0);
   const int mask = g.n1 - 1;
-  // count
   for (int j = 0; j < g.M; j++) {
     int prev = -1;
     for (int t = 0; t < kTaps; t++) {
@@ -202,11 +204,37 @@ void build_bins(const Geometry& g, int C, Bins& b) {
       int t1 = t;
       while (t1 + 1 < kTaps && (((g.iy0[j] + t1 + 1) & mask) / C) == blk) t1++;
       const int col0 = col - blk * C;
-      b.ent[fill[blk]++] = make_int2(j, t | (t1 << 8) | (slot << 16) | (col0 << 24));
+      b.ent[fill[blk]++] = make_int4(j, t | (t1 << 8) | (slot << 16) | (col0 << 24), g.ix0[j], 0);
       slot++;
       t = t1 + 1;
     }
     b.nslot[j] = (int8_t)slot;
+  }
+  const int n0 = g.n0;
+  b.xoff.assign((size_t)b.nblk * (n0 + 1), 0);
+  for (int blk = 0; blk < b.nblk; blk++) {
+    std::stable_sort(b.ent.begin() + b.off[blk], b.ent.begin() + b.off[blk + 1],
+                     [](const int4& u, const int4& v) { return u.z < v.z; });
+    int* xo = &b.xoff[(size_t)blk * (n0 + 1)];
+    int e = b.off[blk];
+    for (int x = 0; x <= n0; x++) {
+      while (e < b.off[blk + 1] && b.ent[e].z < x) e++;
+      xo[x] = e;
+    }
+  }
+  // self-contained records: the node's x-taps, the y-tap under every column
+  // of the tile (zero outside the run) and the node -> entries map
+  const size_t E = b.ent.size();
+  b.px.assign(E * kTaps, 0.f);
+  b.py.assign(E * (size_t)C, 0.f);
+  b.of_node.assign((size_t)g.M * kMaxSlot, -1);
+  for (size_t e = 0; e < E; e++) {
+    const int4 r = b.ent[e];
+    const int j = r.x, t0 = r.y & 0xff, t1 = (r.y >> 8) & 0xff, slot = (r.y >> 16) & 0xff;
+    const int col0 = (r.y >> 24) & 0xff;
+    for (int t = 0; t < kTaps; t++) b.px[e * kTaps + t] = g.phix[(size_t)j * kTaps + t];
+    for (int t = t0; t <= t1; t++) b.py[e * C + col0 + t - t0] = g.phiy[(size_t)j * kTaps + t];
+    b.of_node[(size_t)j * kMaxSlot + slot] = (int)e;
   }
 }
 

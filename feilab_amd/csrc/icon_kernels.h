@@ -36,27 +36,31 @@ struct RowAdjArgs {
 };
 
 struct ColArgs {
-  int S, N0, N1, M, n1;
+  int S, N0, N1, M, n1, E;
   const float2* in;        // fwd: tmp [S][N0][n1]
   float2* out;             // adj: tmp [S][N0][n1]
   const float2* tw;
   const int* ent_off;
-  const int2* ent;
+  const int4* ent;
+  const int* xoff;         // adj: [nblk][n0+1] row offsets into ent
   const float* phix;
   const float* phiy;
   const int* ix0;
+  const float* ent_px;     // [E][14]
+  const float* ent_py;     // [E][C]
   float2* part;            // fwd: [S][M][3]
-  const float2* fw;        // adj: [S][M]
+  const float2* fw;        // adj: [S][E] weighted value per bin entry
 };
 
 struct CombineArgs {
-  int S, M, N0, nz;
+  int S, M, N0, nz, E;
   int mode;              // 0: after A m, 1: after A rhat (step size), 2: initial b, 3: plain sum
   const float2* part;    // [S][M][3]
   const int8_t* nslot;
   const float* w;
   const float2* b;       // [S][M]
-  float2* fw;            // mode 0/2: weighted node values; mode 3: sums
+  float2* fw;            // mode 0/2: [S][E] weighted value per bin entry; mode 3: [S][M] sums
+  const int* ent_of_node;  // [M][3] bin entries of each node
   double* bnorm;         // [S]
   double* resid;         // mode 0: trace base or null
   int resid_stride, resid_index;
@@ -99,7 +103,10 @@ hipError_t launch_fft_rows(int NR, float2* data, int rows, int sign, const float
 hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int sign, const float2* tw,
                            hipStream_t st);
 hipError_t launch_zero(float* p, size_t n, hipStream_t st);
+// fw_ent[s][e] = f[s][node of e]
+hipError_t launch_node_to_ent(const float2* f, float2* fw_ent, const int4* ent, int S, int M, int E, hipStream_t st);
 hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
+hipError_t launch_copy4(float4* dst, const float4* src, size_t n, hipStream_t st);
 bool col_config_ok(int NC, int C);
 hipError_t set_col_attrs(int NC, int C);
 

@@ -141,14 +141,20 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
 }
 
 // --------------------------------------------------------------- column pass
-// One block per (slice, tile of C grid columns); the n0 x C tile lives in LDS
-// (row-major, C fastest).  Thread (c = tid % C, t = tid / C) works on column c.
-
+// One block per (slice, tile of C grid columns); the n0 x C tile lives in LDS,
+// row-major with a padded row stride of C+1 float2 so that rows map to
+// different banks.  FFT: thread (c = tid % C, t = tid / C) works on column c.
+// Interpolation: 16-lane groups take one bin entry (node, run of y-taps
+// inside the tile) each; lane tx < 14 reads the C cells of grid row
+// x0 + tx and the group sums its 14 rows with a 16-lane shuffle reduction.
+// Spread: see k_col_adj (row-owner gather, no atomics).
 
 template <int NC, int C>
 __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   constexpr int NT = C * (NC / 16);
-  extern __shared__ float2 tile[];   // NC * C
+  constexpr int LS = C + 1;
+  constexpr int NG = NT / 16;
+  extern __shared__ float2 tile[];   // NC * LS
   const int nblk = a.n1 / C;
   const int s = blockIdx.x / nblk;
   const int blk = blockIdx.x % nblk;
@@ -159,87 +165,155 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   for (int i = tid; i < NC * C; i += NT) {
     const int x = i / C, c = i % C;
     const int X = (x + h0) & (NC - 1);
-    tile[i] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
+    tile[x * LS + c] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<NC, C, -1>(tile + (tid % C), tid / C, a.tw);
+  fft_lds<NC, LS, -1>(tile + (tid % C), tid / C, a.tw);
   __syncthreads();
   // interpolation partials of this tile's nodes
+  const int lane = tid & 15;
+  const int grp = tid >> 4;
   const int e1 = a.ent_off[blk + 1];
-  for (int e = a.ent_off[blk] + tid; e < e1; e += NT) {
-    const int2 en = a.ent[e];
-    const int j = en.x;
-    const int t0 = en.y & 0xff, t1 = (en.y >> 8) & 0xff, slot = (en.y >> 16) & 0xff;
-    const int col0 = (en.y >> 24) & 0xff;
-    const int ixs = a.ix0[j];
-    float px[kTapsK];
+  for (int e = a.ent_off[blk] + grp; e < e1; e += NG) {
+    const int4 en = a.ent[e];
+    // lane q < 14: the node's x-tap q; every lane: the y-taps of the C columns
+    const float px = lane < kTapsK ? a.ent_px[(size_t)e * kTapsK + lane] : 0.f;
+    float py[C];
+    const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
 #pragma unroll
-    for (int tx = 0; tx < kTapsK; tx++) px[tx] = a.phix[(size_t)j * kTapsK + tx];
-    float sx = 0.f, sy = 0.f;
-    for (int ty = t0; ty <= t1; ty++) {
-      const float py = a.phiy[(size_t)j * kTapsK + ty];
-      const int c = col0 + ty - t0;
-      float ax = 0.f, ay = 0.f;
+    for (int c4 = 0; c4 < C / 4; c4++) {
+      const float4 v = pyv[c4];
+      py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
+    }
+    float ax = 0.f, ay = 0.f;
+    if (lane < kTapsK) {
+      const float2* row = tile + ((en.z + lane) & (NC - 1)) * LS;
 #pragma unroll
-      for (int tx = 0; tx < kTapsK; tx++) {
-        const int x = (ixs + tx) & (NC - 1);
-        const float phi = px[tx] * py;
-        const float2 gv = tile[x * C + c];
+      for (int c = 0; c < C; c++) {
+        const float phi = px * py[c];
+        const float2 gv = row[c];
         ax = __fmaf_rn(gv.x, phi, ax);
         ay = __fmaf_rn(gv.y, phi, ay);
       }
-      sx += ax;
-      sy += ay;
     }
-    a.part[((size_t)s * a.M + j) * kMaxSlotK + slot] = make_float2(sx, sy);
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) {
+      ax += __shfl_xor(ax, o, 16);
+      ay += __shfl_xor(ay, o, 16);
+    }
+    if (lane == 0) a.part[((size_t)s * a.M + en.x) * kMaxSlotK + ((en.y >> 16) & 0xff)] = make_float2(ax, ay);
   }
+}
+
+// Wavefront-scope barrier for LDS data exchanged between the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int NC, int C>
 __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
   constexpr int NT = C * (NC / 16);
-  extern __shared__ float2 tile[];   // NC * C
+  constexpr int LS = C + 1;
+  constexpr int NW = NT / 64;                    // waves per block
+  constexpr int RPT = NC >= NT ? NC / NT : 1;    // grid rows per thread
+  constexpr int REC = kTapsK + C + 3;            // staged record: x-taps, y-taps, f, first x-tap
+  constexpr int REGION = NC * LS * 2 / NW;       // floats of LDS staging per wave
+  static_assert(64 * REC <= REGION, "staging chunk does not fit");
+  extern __shared__ float2 tile[];   // NC * LS
   const int nblk = a.n1 / C;
   const int s = blockIdx.x / nblk;
   const int blk = blockIdx.x % nblk;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
-  for (int i = tid; i < NC * C; i += NT) tile[i] = make_float2(0.f, 0.f);
-  __syncthreads();
-  // spread this tile's nodes into LDS
-  float* tf = reinterpret_cast<float*>(tile);
-  const int e1 = a.ent_off[blk + 1];
-  for (int e = a.ent_off[blk] + tid; e < e1; e += NT) {
-    const int2 en = a.ent[e];
-    const int j = en.x;
-    const int t0 = en.y & 0xff, t1 = (en.y >> 8) & 0xff;
-    const int col0 = (en.y >> 24) & 0xff;
-    const int ixs = a.ix0[j];
-    const float2 f = a.fw[(size_t)s * a.M + j];
-    float px[kTapsK];
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  // Spread as a gather.  Lane x of a wave owns grid row x of the tile (C
+  // cells in registers).  The entries whose 14-row x-window can cover the
+  // wave's 64 rows [xa, xa+64) are the contiguous runs of the tile's x-sorted
+  // list with first x-tap in [xa-13, xa+63] (mod n0); the wave stages them,
+  // 64 at a time, in its share of the (not yet used) tile LDS, and each lane
+  // adds the staged entries that cover its row.  No atomics; every cell sums
+  // its entries in list order.
+  const int* xo = a.xoff + (size_t)blk * (NC + 1);
+  const float2* fw = a.fw + (size_t)s * a.E;
+  float* stage = reinterpret_cast<float*>(tile) + wave * REGION;
+  float2 acc[RPT][C];
 #pragma unroll
-    for (int tx = 0; tx < kTapsK; tx++) px[tx] = a.phix[(size_t)j * kTapsK + tx];
-    for (int ty = t0; ty <= t1; ty++) {
-      const float py = a.phiy[(size_t)j * kTapsK + ty];
-      const int c = col0 + ty - t0;
+  for (int r = 0; r < RPT; r++) {
 #pragma unroll
-      for (int tx = 0; tx < kTapsK; tx++) {
-        const int x = (ixs + tx) & (NC - 1);
-        const float phi = px[tx] * py;
-        atomicAdd(&tf[2 * (x * C + c)], f.x * phi);
-        atomicAdd(&tf[2 * (x * C + c) + 1], f.y * phi);
+    for (int c = 0; c < C; c++) acc[r][c] = make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int xa = r * NT + wave * 64;
+    if (xa >= NC) break;
+    const int x = xa + lane;
+    int lo1, hi1, lo2 = 0, hi2 = 0;
+    if (xa >= kTapsK - 1) {
+      lo1 = xo[xa - (kTapsK - 1)]; hi1 = xo[xa + 64];
+    } else {
+      lo1 = xo[NC - (kTapsK - 1) + xa]; hi1 = xo[NC]; lo2 = xo[0]; hi2 = xo[xa + 64];
+    }
+    for (int run = 0; run < 2; run++) {
+      const int lo = run ? lo2 : lo1, hi = run ? hi2 : hi1;
+      for (int base = lo; base < hi; base += 64) {
+        const int cnt = min(64, hi - base);
+        wave_sync();
+        if (lane < cnt) {
+          const int e = base + lane;
+          float* rec = stage + lane * REC;
+#pragma unroll
+          for (int q = 0; q < kTapsK; q++) rec[q] = a.ent_px[(size_t)e * kTapsK + q];
+          const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
+#pragma unroll
+          for (int c4 = 0; c4 < C / 4; c4++) {
+            const float4 v = pyv[c4];
+            rec[kTapsK + 4 * c4] = v.x; rec[kTapsK + 4 * c4 + 1] = v.y;
+            rec[kTapsK + 4 * c4 + 2] = v.z; rec[kTapsK + 4 * c4 + 3] = v.w;
+          }
+          const float2 f = fw[e];
+          rec[kTapsK + C] = f.x;
+          rec[kTapsK + C + 1] = f.y;
+          rec[kTapsK + C + 2] = __int_as_float(a.ent[e].z);
+        }
+        wave_sync();
+        for (int k = 0; k < cnt; k++) {
+          const float* rec = stage + k * REC;
+          const int tx = (x - __float_as_int(rec[kTapsK + C + 2])) & (NC - 1);
+          if (tx < kTapsK) {
+            const float px = rec[tx];
+            const float fx = rec[kTapsK + C], fy = rec[kTapsK + C + 1];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+              const float phi = px * rec[kTapsK + c];
+              acc[r][c].x = __fmaf_rn(fx, phi, acc[r][c].x);
+              acc[r][c].y = __fmaf_rn(fy, phi, acc[r][c].y);
+            }
+          }
+        }
       }
     }
   }
   __syncthreads();
-  fft_lds<NC, C, 1>(tile + (tid % C), tid / C, a.tw);
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int x = r * NT + wave * 64 + lane;
+    if (x < NC) {
+#pragma unroll
+      for (int c = 0; c < C; c++) tile[x * LS + c] = acc[r][c];
+    }
+  }
+  __syncthreads();
+  fft_lds<NC, LS, 1>(tile + (tid % C), tid / C, a.tw);
   __syncthreads();
   // keep the N0 rows the crop needs
   const int h0 = a.N0 >> 1;
   for (int i = tid; i < a.N0 * C; i += NT) {
     const int X = i / C, c = i % C;
     const int x = (X - h0) & (NC - 1);
-    a.out[((size_t)s * a.N0 + X) * a.n1 + y0 + c] = tile[x * C + c];
+    a.out[((size_t)s * a.N0 + X) * a.n1 + y0 + c] = tile[x * LS + c];
   }
 }
 
@@ -291,12 +365,20 @@ __global__ __launch_bounds__(1024) void k_combine(CombineArgs a) {
         const int i = j / a.N0;
         if (i == a.iomit) a.frep[(size_t)s * a.N0 + (j - i * a.N0)] = f;
       }
-      a.fw[(size_t)s * a.M + j] = make_float2(f.x * wj, f.y * wj);
+      const float2 v = make_float2(f.x * wj, f.y * wj);
+      for (int q = 0; q < kMaxSlotK; q++) {
+        const int e = a.ent_of_node[(size_t)j * kMaxSlotK + q];
+        if (e >= 0) a.fw[(size_t)s * a.E + e] = v;
+      }
     } else if (a.mode == 1) {
       acc += (double)f.x * (double)f.x * (double)wj + (double)f.y * (double)f.y * (double)wj;
     } else if (a.mode == 2) {
       acc += (double)wj * ((double)f.x * (double)f.x + (double)f.y * (double)f.y);
-      a.fw[(size_t)s * a.M + j] = make_float2(f.x * wj, f.y * wj);
+      const float2 v = make_float2(f.x * wj, f.y * wj);
+      for (int q = 0; q < kMaxSlotK; q++) {
+        const int e = a.ent_of_node[(size_t)j * kMaxSlotK + q];
+        if (e >= 0) a.fw[(size_t)s * a.E + e] = v;
+      }
     } else {
       a.fw[(size_t)s * a.M + j] = f;
     }
@@ -448,13 +530,23 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_fft_cols(float2* data, int n1
   const int y0 = (blockIdx.x % nblk) * C;
   float2* base = data + (size_t)g * NC * n1;
   const int tid = threadIdx.x;
-  for (int i = tid; i < NC * C; i += NT) tile[i] = base[(size_t)(i / C) * n1 + y0 + i % C];
+  constexpr int LS = C + 1;
+  for (int i = tid; i < NC * C; i += NT) tile[(i / C) * LS + i % C] = base[(size_t)(i / C) * n1 + y0 + i % C];
   __syncthreads();
-  if (sign < 0) fft_lds<NC, C, -1>(tile + (tid % C), tid / C, tw);
-  else fft_lds<NC, C, 1>(tile + (tid % C), tid / C, tw);
+  if (sign < 0) fft_lds<NC, LS, -1>(tile + (tid % C), tid / C, tw);
+  else fft_lds<NC, LS, 1>(tile + (tid % C), tid / C, tw);
   __syncthreads();
-  for (int i = tid; i < NC * C; i += NT) base[(size_t)(i / C) * n1 + y0 + i % C] = tile[i];
+  for (int i = tid; i < NC * C; i += NT) base[(size_t)(i / C) * n1 + y0 + i % C] = tile[(i / C) * LS + i % C];
   (void)count;
+}
+
+__global__ void k_node_to_ent(const float2* __restrict__ f, float2* __restrict__ fw, const int4* __restrict__ ent,
+                              int S, int M, int E) {
+  const size_t n = (size_t)S * E;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(i / E), e = (int)(i % E);
+    fw[i] = f[(size_t)s * M + ent[e].x];
+  }
 }
 
 __global__ void k_zero_f32(float* p, size_t n) {
@@ -463,6 +555,12 @@ __global__ void k_zero_f32(float* p, size_t n) {
 }
 
 __global__ void k_copy_f32(float* dst, const float* src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// Plain 16-byte vector copy (bandwidth / counter calibration).
+__global__ __launch_bounds__(256) void k_copy_f4(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
 }
@@ -562,7 +660,7 @@ hipError_t launch_fft_rows(int NR, float2* data, int rows, int sign, const float
 }
 
 // Column kernels: (NC, C) with NC * C * 8 bytes of LDS.
-#define COL_CASES(X) X(128, 16) X(128, 32) X(256, 8) X(256, 16) X(256, 32) X(512, 8) X(512, 16) X(512, 32) X(1024, 8) X(1024, 16) X(2048, 8)
+#define COL_CASES(X) X(128, 8) X(128, 16) X(256, 8) X(256, 16) X(512, 8) X(512, 16) X(1024, 8) X(1024, 16) X(2048, 8)
 
 bool col_config_ok(int NC, int C) {
 #define CASE(N, CC) if (NC == N && C == CC) return true;
@@ -576,13 +674,13 @@ static hipError_t col_attr() {
   static bool done = false;
   if (!done) {
     hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       NC * C * (int)sizeof(float2));
+                                       NC * (C + 1) * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_col_adj<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            NC * C * (int)sizeof(float2));
+                            NC * (C + 1) * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_fft_cols<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            NC * C * (int)sizeof(float2));
+                            NC * (C + 1) * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     done = true;
   }
@@ -606,10 +704,10 @@ hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t
     e = col_attr<N, CC>();                                                                               \
     if (e != hipSuccess) return e;                                                                       \
     if (adjoint)                                                                                         \
-      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * CC * sizeof(float2), \
+      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (CC + 1) * sizeof(float2), \
                          st, a);                                                                         \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * CC * sizeof(float2), \
+      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (CC + 1) * sizeof(float2), \
                          st, a);                                                                         \
     return hipGetLastError();                                                                            \
   }
@@ -625,7 +723,7 @@ hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int s
   if (NC == N && C == CC) {                                                                             \
     hipError_t e = col_attr<N, CC>();                                                                   \
     if (e != hipSuccess) return e;                                                                      \
-    hipLaunchKernelGGL((k_fft_cols<N, CC>), dim3(count * nblk), dim3(CC * (N / 16)), N * CC * sizeof(float2), \
+    hipLaunchKernelGGL((k_fft_cols<N, CC>), dim3(count * nblk), dim3(CC * (N / 16)), N * (CC + 1) * sizeof(float2), \
                        st, data, n1, count, sign, tw);                                                  \
     return hipGetLastError();                                                                           \
   }
@@ -639,10 +737,25 @@ hipError_t launch_combine(const CombineArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_node_to_ent(const float2* f, float2* fw_ent, const int4* ent, int S, int M, int E, hipStream_t st) {
+  const size_t n = (size_t)S * E;
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_node_to_ent, dim3(blocks), dim3(256), 0, st, f, fw_ent, ent, S, M, E);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(float* p, size_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_zero_f32, dim3(blocks), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy4(float4* dst, const float4* src, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 32);
+  hipLaunchKernelGGL(k_copy_f4, dim3(blocks), dim3(256), 0, st, dst, src, n);
   return hipGetLastError();
 }
 

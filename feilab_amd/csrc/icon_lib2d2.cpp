@@ -43,6 +43,53 @@ extern "C" int icon_read_tilts(const char* path, float* tilt_deg, int cap, int* 
   return n > cap ? ICON_ERR_CAPACITY : ICON_OK;
 }
 
+extern "C" int icon_mrc_read_y_slices(const char* path, int y0, int count, float* out, int* dims4) {
+  if (!path) return ICON_ERR_ARG;
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) {
+    set_err(std::string("cannot open ") + path);
+    return ICON_ERR_IO;
+  }
+  MrcHeader h;
+  std::string err;
+  int rc = ICON_OK;
+  if (!mrc::read_header(fp, &h)) {
+    set_err("cannot read MRC header");
+    rc = ICON_ERR_IO;
+  } else {
+    if (dims4) { dims4[0] = h.nx; dims4[1] = h.ny; dims4[2] = h.nz; dims4[3] = h.mode; }
+    if (out && count > 0 && !mrc::read_y_slices(fp, h, y0, count, out, &err)) {
+      set_err(err);
+      rc = ICON_ERR_IO;
+    }
+  }
+  std::fclose(fp);
+  return rc;
+}
+
+extern "C" int icon_mrc_write_slice(const char* path, const char* template_path, const float* m, int width,
+                                    int thickness) {
+  if (!path || !template_path || !m) return ICON_ERR_ARG;
+  FILE* fp = std::fopen(template_path, "rb");
+  if (!fp) {
+    set_err(std::string("cannot open ") + template_path);
+    return ICON_ERR_IO;
+  }
+  MrcHeader h;
+  const bool ok = mrc::read_header(fp, &h);
+  std::fclose(fp);
+  if (!ok) {
+    set_err("cannot read MRC header");
+    return ICON_ERR_IO;
+  }
+  std::string err;
+  if (!mrc::write_icon_slice(path, h, m, width, thickness, &err)) {
+    set_err(err);
+    return ICON_ERR_IO;
+  }
+  return ICON_OK;
+}
+
 extern "C" const char* icon_lib_2d2_error(void) {
   std::lock_guard<std::mutex> lk(g_lmu);
   return g_lerr_store.c_str();

@@ -36,8 +36,14 @@ struct Bins {
   int C = 16;                     // grid columns per tile
   int nblk = 0;                   // n1 / C
   std::vector<int> off;           // nblk+1
-  std::vector<int2> ent;          // (node j, t0 | t1<<8 | slot<<16 | col0<<24)
+  // (node j, t0 | t1<<8 | slot<<16 | col0<<24, first x-tap x0, 0); a tile's
+  // entries are sorted by x0 (then node)
+  std::vector<int4> ent;
+  std::vector<int> xoff;          // nblk*(n0+1): first entry of the tile with x0 >= x
   std::vector<int8_t> nslot;      // M
+  std::vector<float> px;          // E*14: x-taps of the entry's node
+  std::vector<float> py;          // E*C: y-tap of each tile column (0 outside the run)
+  std::vector<int> of_node;       // M*3: entries of node j by slot (-1: none)
 };
 
 // Fills g; returns false (with a message) on an invalid geometry.
@@ -80,7 +86,11 @@ struct icon_plan {
   float* d_phiy = nullptr;
   int* d_ix0 = nullptr;
   int* d_ent_off = nullptr;
-  int2* d_ent = nullptr;
+  int4* d_ent = nullptr;
+  int* d_xoff = nullptr;
+  float* d_ent_px = nullptr;
+  float* d_ent_py = nullptr;
+  int* d_ent_of_node = nullptr;
   int8_t* d_nslot = nullptr;
   float* d_phi1 = nullptr;
   int* d_i10 = nullptr;
@@ -88,7 +98,7 @@ struct icon_plan {
   const float2* d_tw1 = nullptr;    // twiddles for n1
   // per-slice work buffers (capacity S_max)
   float2* d_b = nullptr;            // [S][M] Fourier-slice data
-  float2* d_fw = nullptr;           // [S][M] weighted node values
+  float2* d_fw = nullptr;           // [S][E] weighted node value of every bin entry
   float2* d_part = nullptr;         // [S][M][kMaxSlot] interpolation partials
   float2* d_ahwb = nullptr;         // [S][N0][N1]
   float2* d_rhat = nullptr;         // [S][N0][N1]

@@ -92,6 +92,11 @@ def lib():
                          ctypes.POINTER(ci), ctypes.POINTER(cf), ci],
         "icon_read_tilts": [ctypes.c_char_p, ctypes.POINTER(cf), ci, ctypes.POINTER(ci)],
         "icon_partition_slices": [ci, ctypes.POINTER(ci), ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)],
+        "icon_geometry_tables": [ci, ci, ci, ctypes.POINTER(cf), ctypes.POINTER(PlanInfo)] + [vp] * 10,
+        "icon_mrc_read_y_slices": [ctypes.c_char_p, ci, ci, vp, ctypes.POINTER(ci)],
+        "icon_mrc_write_slice": [ctypes.c_char_p, ctypes.c_char_p, vp, ci, ci],
+        "icon_time_kernel": [vp, ci, ci, ci, ctypes.POINTER(ctypes.c_double)],
+        "icon_copy_probe": [vp, vp, cs, ci, ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -221,6 +226,15 @@ class Plan:
                                       _ptr(d_resid), _ptr(d_alpha), ctypes.byref(opts), _ptr(stream)),
                "icon_reconstruct")
 
+    KERNELS = {"row_fwd": 0, "col_fwd": 1, "combine": 2, "col_adj": 3, "row_adj": 4, "sino": 5}
+
+    def time_kernel(self, name, S, reps=20):
+        """Average launch time (ms) of one iteration kernel over S slices (HIP events on the plan stream)."""
+        ms = ctypes.c_double(0)
+        _check(lib().icon_time_kernel(self.h, self.KERNELS[name], int(S), int(reps), ctypes.byref(ms)),
+               "icon_time_kernel")
+        return ms.value
+
     # -- host conveniences ---------------------------------------------------
     def forward(self, img):
         img = np.asarray(img)
@@ -289,6 +303,58 @@ def fft_2d(x, sign=-1):
     d = DeviceArray.from_host(x)
     _check(lib().icon_fft_2d(n0, n1, x.size // (n0 * n1), sign, d.ptr, None), "icon_fft_2d")
     return d.numpy()
+
+
+def copy_probe(nbytes=1 << 30, reps=20):
+    """HBM bandwidth of a plain device copy: returns (ms per copy, GB/s counting read + write)."""
+    src = DeviceArray((nbytes // 4,), np.float32)
+    dst = DeviceArray((nbytes // 4,), np.float32)
+    ms = ctypes.c_double(0)
+    _check(lib().icon_copy_probe(dst.ptr, src.ptr, nbytes, reps, ctypes.byref(ms)), "icon_copy_probe")
+    return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+def geometry_tables(nx, tilt_deg, thickness=0):
+    """Host-side plan geometry (no device): the tables icon_plan_create uploads."""
+    tilt = np.ascontiguousarray(tilt_deg, np.float32)
+    nz = len(tilt)
+    info = PlanInfo()
+    _check(lib().icon_geometry_tables(nx, thickness, nz, tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                      ctypes.byref(info), *([None] * 10)), "icon_geometry_tables")
+    N0, N1, M = info.N0, info.N1, info.M
+    out = dict(nodes=np.zeros((M, 2), np.float32), w=np.zeros(M, np.float32),
+               phix=np.zeros((M, 14), np.float32), phiy=np.zeros((M, 14), np.float32),
+               ix0=np.zeros(M, np.int32), iy0=np.zeros(M, np.int32), ck=np.zeros((N0, N1), np.float32),
+               ck1=np.zeros(N0, np.float32), phi1=np.zeros((N0, 14), np.float32), i10=np.zeros(N0, np.int32))
+    order = ["nodes", "w", "phix", "phiy", "ix0", "iy0", "ck", "ck1", "phi1", "i10"]
+    _check(lib().icon_geometry_tables(nx, thickness, nz, tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                      ctypes.byref(info), *[out[k].ctypes.data_as(ctypes.c_void_p) for k in order]),
+           "icon_geometry_tables")
+    out["info"] = info
+    return out
+
+
+def mrc_read_y_slices(path, y0, count):
+    """Slices [y0, y0+count) of an MRC tilt series as [count][nz][nx] float32 (mrc_read_slice 'y')."""
+    dims = (ctypes.c_int * 4)()
+    rc = lib().icon_mrc_read_y_slices(path.encode(), 0, 0, None, dims)
+    if rc:
+        raise IconError(rc, lib().icon_lib_2d2_error().decode(errors="replace"))
+    nx, ny, nz = dims[0], dims[1], dims[2]
+    out = np.zeros((count, nz, nx), np.float32)
+    rc = lib().icon_mrc_read_y_slices(path.encode(), y0, count, out.ctypes.data_as(ctypes.c_void_p), dims)
+    if rc:
+        raise IconError(rc, lib().icon_lib_2d2_error().decode(errors="replace"))
+    return out
+
+
+def mrc_write_slice(path, template_path, m):
+    """Writes a reconstructed slice m[width][thickness] as ICON's midYYYYY.mrc."""
+    m = np.ascontiguousarray(m, np.float32)
+    rc = lib().icon_mrc_write_slice(path.encode(), template_path.encode(), m.ctypes.data_as(ctypes.c_void_p),
+                                    m.shape[0], m.shape[1])
+    if rc:
+        raise IconError(rc, lib().icon_lib_2d2_error().decode(errors="replace"))
 
 
 def read_tilts(path, cap=1000):

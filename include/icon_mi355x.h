@@ -129,6 +129,47 @@ int icon_reconstruct(icon_plan* plan, int S, const void* d_sino, void* d_m, cons
 int icon_reconstruct_host(icon_plan* plan, int S, const float* sino, float* m, const float* m_init,
                           float* reproj, double* resid, double* alpha, const icon_solve_opts* opts);
 
+/* ---- host-only helpers (no device needed) -------------------------------- */
+/* The plan geometry computed on the host exactly as icon_plan_create does
+ * (MainCode.cu:38-78, weight.cu:8-41, px2D.cu:8-47, nufft_gpu_v8.cu:22-42,
+ * 221-235, 1242-1271).  Any output pointer may be NULL.  Sizes: nodes 2*M,
+ * weights M, phix/phiy M*14, ix0/iy0 M (first wrapped tap index), ck N0*N1,
+ * ck1 N0, phi1 N0*14, i10 N0; info as icon_plan_info (device fields 0). */
+int icon_geometry_tables(int nx, int thickness, int nz, const float* tilt_deg, icon_plan_info* info,
+                         float* nodes, float* weights, float* phix, float* phiy, int* ix0, int* iy0,
+                         float* ck, float* ck1, float* phi1, int* i10);
+
+/* MRC I/O of the command line (mrcfile3.c in the reference).
+ * Reads slices [y0, y0+count) of a tilt series into out[count][nz][nx]
+ * (mrc_read_slice 'y', mrcfile3.c:902-951) and reports nx, ny, nz, mode. */
+int icon_mrc_read_y_slices(const char* path, int y0, int count, float* out, int* dims4);
+/* Writes one reconstructed slice as saveMRC_real_inv + mrc_update_head do
+ * (saveMRC_real.cu:33-56, mrcfile3.c:113-514): header copied from the tilt
+ * series `template_path`, image m[width][thickness] transposed to x fastest. */
+int icon_mrc_write_slice(const char* path, const char* template_path, const float* m, int width,
+                         int thickness);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Kernels of one ICON iteration, for icon_time_kernel. */
+typedef enum {
+  ICON_K_ROW_FWD = 0,   /* deapodise + pad + y-FFT of the image (fused update) */
+  ICON_K_COL_FWD = 1,   /* x-FFT of the grid tile + interpolation to the nodes */
+  ICON_K_COMBINE = 2,   /* node sums, weights, residual (FP64)                 */
+  ICON_K_COL_ADJ = 3,   /* spread of the nodes into the grid tile + x-IFFT     */
+  ICON_K_ROW_ADJ = 4,   /* y-IFFT + crop + deapodise + gradient + |r|^2 (FP64) */
+  ICON_K_SINO = 5       /* 1D transform of the projection rows                 */
+} icon_kernel_id;
+
+/* Times `reps` back-to-back launches of one kernel of the iteration over S
+ * slices, with HIP events on the plan's stream, on the buffers of the plan's
+ * last solve; *avg_ms = average launch duration.  Not part of the
+ * reference; measurement only. */
+int icon_time_kernel(icon_plan* plan, int kernel, int S, int reps, double* avg_ms);
+
+/* Device-to-device copy of `bytes` repeated `reps` times with a plain vector
+ * copy kernel; *avg_ms per copy.  Calibrates HBM bandwidth and counters. */
+int icon_copy_probe(void* d_dst, const void* d_src, size_t bytes, int reps, double* avg_ms);
+
 /* ---- file-level entry point -------------------------------------------- */
 /* Lib_2D2 (MainCode.cu:3-177): reconstruct slices [slice_begin, slice_end)
  * of the MRC tilt series `mrc_path` with the tilt file `angle_path`, writing

@@ -1,0 +1,205 @@
+"""Pins the CPU oracle (the CPU restatement of the reference algorithm) with
+checks that do not restate its formulas: exact transforms, closed forms,
+invariants and special cases that reduce to textbook routines.  Each check is
+chosen so a plausible slip (a dropped factor, a wrong sign or index, a
+transposed operand) fails it.  CPU only.
+"""
+import numpy as np
+import pytest
+
+from feilab_amd import phantom
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+# ---------------------------------------------------------------- geometry
+
+def test_grid_is_twice_next_power_of_two(oracle_mod):
+    """MainCode.cu:66-71 in float arithmetic gives n = 2 nextpow2(N) for every width."""
+    for N in list(range(16, 1100)) + [2047, 2048, 4095, 4096]:
+        s, n = oracle_mod.grid(N)
+        assert n == 2 * (1 << int(np.ceil(np.log2(N)))), N
+        assert abs(s - n / N) < 1e-6 * s
+
+
+def test_window_shape_closed_form(oracle_mod):
+    for sigma in (2.0, 2.56, 3.76):
+        b, bf = oracle_mod.window_params(sigma)
+        assert b == pytest.approx(2 * sigma * 6 / ((2 * sigma - 1) * np.pi), rel=1e-6)
+        assert bf == pytest.approx(1 / np.sqrt(np.pi * b), rel=1e-6)
+
+
+def test_weights_uniform_spacing_properties(oracle_mod):
+    """For uniform angular spacing d every tilt row is the same ramp 2 d |j - c|
+    (ends: 2|d|, interior: |2d|), with 180/(4 nz) at the centre."""
+    tilt = -phantom.tilt_series(60, 3)          # negated angles, as the reference uses them
+    w = oracle_mod.weight(64, tilt)
+    j = np.arange(64)
+    ramp = 2 * 3.0 * np.abs(j - 32)
+    ramp[32] = 180.0 / (4 * len(tilt))
+    for i in range(len(tilt)):
+        np.testing.assert_allclose(w[i], ramp, rtol=1e-6)
+
+
+def test_weights_nonuniform_use_neighbour_span(oracle_mod):
+    th = np.array([-60, -50, -45, -30, 0, 40], np.float32)
+    w = oracle_mod.weight(16, th)
+    c = 8
+    assert w[0, 0] == pytest.approx(2 * 10 * 8)      # end: twice the gap to the neighbour
+    assert w[5, 0] == pytest.approx(2 * 40 * 8)
+    assert w[2, 3] == pytest.approx(20 * 5)          # interior: |theta_{i-1} - theta_{i+1}|
+    assert w[3, c] == pytest.approx(180.0 / 24)
+
+
+def test_radial_nodes_on_lines_through_origin(oracle_mod):
+    th = np.array([-60, -13.5, 0, 7, 45], np.float32)
+    N = 37
+    xy = oracle_mod.px2d(N, th).reshape(len(th), N, 2)
+    f = -0.5 + np.arange(N) / N
+    for i, t in enumerate(th):
+        r = np.hypot(xy[i, :, 0], xy[i, :, 1])
+        np.testing.assert_allclose(r, np.abs(f), atol=2e-6)
+        ang = np.rad2deg(np.arctan2(xy[i, :, 1], xy[i, :, 0]))
+        k = np.abs(f) > 0.05
+        # direction t (radius > 0) or t - 180 (radius < 0)
+        d = (ang[k] - t + 90) % 180 - 90
+        np.testing.assert_allclose(d, 0, atol=1e-3)
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [16, 256, 1024])
+def test_fft_matches_numpy(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel(oracle_mod.fft(x, -1, 64), np.fft.fft(x)) < 1e-13
+    assert rel(oracle_mod.fft(x, +1, 64), np.fft.ifft(x) * n) < 1e-13
+    assert rel(oracle_mod.fft(x, -1, 32), np.fft.fft(x)) < 2e-6
+    g = rng.standard_normal((8, 32)) + 1j * rng.standard_normal((8, 32))
+    assert rel(oracle_mod.fft2(g, -1, 64), np.fft.fft2(g)) < 1e-13
+
+
+# ---------------------------------------------------------------- NUFFT
+
+@pytest.mark.parametrize("prec,tol", [(64, 1.5e-5), (32, 1.5e-5)])
+def test_nufft_matches_exact_ndft(oracle_mod, prec, tol):
+    """Gaussian window, m = 6, sigma = 2: error bound 4 exp(-m pi (1 - 1/(2 sigma - 1))) = 1.4e-5."""
+    rng = np.random.default_rng(1)
+    tilt = phantom.tilt_series(60, 6)
+    P = oracle_mod.Plan(32, 32, tilt, prec)
+    nodes = P.tables()["nodes"]
+    img = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    assert rel(P.forward(img), oracle_mod.ndft_forward(nodes, img)) < tol
+    f = rng.standard_normal(P.M) + 1j * rng.standard_normal(P.M)
+    assert rel(P.adjoint(f), oracle_mod.ndft_adjoint(nodes, f, 32, 32)) < tol
+
+
+def test_nufft_adjointness_fp64(oracle_mod):
+    rng = np.random.default_rng(2)
+    P = oracle_mod.Plan(40, 24, phantom.tilt_series(50, 5), 64)
+    x = rng.standard_normal((40, 24)) + 1j * rng.standard_normal((40, 24))
+    y = rng.standard_normal(P.M) + 1j * rng.standard_normal(P.M)
+    lhs = np.vdot(y, P.forward(x))
+    rhs = np.vdot(P.adjoint(y), x)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_zero_tilt_is_dft_of_column_sums(oracle_mod):
+    """At theta = 0 the nodes are (f, 0): the 2D transform reduces to the 1D DFT of
+    the sums over the depth axis (Fourier-slice theorem)."""
+    rng = np.random.default_rng(3)
+    N = 64
+    P = oracle_mod.Plan(N, N, np.array([0.0, 30.0], np.float32), 64)
+    img = rng.standard_normal((N, N))
+    f = P.forward(img.astype(complex))[:N]
+    colsum = img.sum(1)
+    k = -0.5 + np.arange(N) / N
+    X = np.arange(N) - N // 2
+    exact = np.exp(-2j * np.pi * np.outer(k, X)) @ colsum
+    assert rel(f, exact) < 1.5e-5
+
+
+def test_projection_transform_is_centred_dft(oracle_mod):
+    """Equispaced nodes (k - N/2)/N: the 1D NUFFT of a projection row is the
+    centred DFT b_k = sum_X p_X exp(-2 pi i (X - N/2)(k - N/2)/N)."""
+    rng = np.random.default_rng(4)
+    N = 100
+    P = oracle_mod.Plan(N, N, np.array([-10.0, 10.0], np.float32), 64)
+    p = rng.standard_normal((2, N)).astype(np.float32)
+    b = P.sino_to_b(p)[:N]
+    X = np.arange(N) - N // 2
+    exact = np.exp(-2j * np.pi * np.outer(X, X) / N) @ p[0].astype(float)
+    assert rel(b, exact) < 1.5e-5
+
+
+# ---------------------------------------------------------------- solver
+
+def _case(seed=5, N=48, step=4, snr=None):
+    rng = np.random.default_rng(seed)
+    tilt = phantom.tilt_series(60, step)
+    b = phantom.blobs_2d(N, 8, rng)
+    sino = phantom.project_2d(b, N, tilt)
+    if snr:
+        sino = phantom.add_noise(sino, snr, rng)
+    return tilt, np.asarray(sino, np.float32), b
+
+
+def test_step_size_is_exact_line_search(oracle_mod):
+    """alpha = |rhat|^2 / sum w |A rhat|^2 minimises J(m - alpha rhat) for the complex
+    step: recompute the first step from the operators and compare with the trace."""
+    tilt, sino, _ = _case()
+    P = oracle_mod.Plan(48, 48, tilt, 64)
+    _, _, al = P.solve(sino, 1, 0, 0, 2, 0.0, trace=True)
+    t = P.tables()
+    b = P.sino_to_b(sino)
+    w = t["w"].astype(float)
+    rhat = -P.adjoint(w * b)                 # gradient at m = 0
+    Ar = P.forward(rhat)
+    alpha = np.vdot(rhat, rhat).real / np.sum(w * np.abs(Ar) ** 2)
+    assert al[0] == pytest.approx(alpha, rel=1e-10)
+    # and it is the minimiser: J along the step is stationary there
+    J = lambda a: np.sum(w * np.abs(P.forward(-a * rhat) - b) ** 2)
+    assert J(alpha) < J(alpha * 1.01) and J(alpha) < J(alpha * 0.99)
+
+
+@pytest.mark.parametrize("dt,thr", [(2, 0.0), (1, 0.0)])
+def test_threshold_bounds_hold(oracle_mod, dt, thr):
+    tilt, sino, _ = _case(snr=1.0)
+    P = oracle_mod.Plan(48, 48, tilt, 32)
+    m = P.solve(sino, 2, 10, 0, dt, thr)
+    if dt == 2:
+        assert m.min() >= thr
+    else:
+        assert m.max() <= thr
+
+
+def test_infr_decreases_residual_and_zero_data_stays_zero(oracle_mod):
+    tilt, sino, _ = _case()
+    P = oracle_mod.Plan(48, 48, tilt, 64)
+    _, res, _ = P.solve(sino, 30, 0, 0, 2, 0.0, trace=True)
+    assert np.all(np.diff(res) <= 1e-12)
+    assert res[-1] < 0.2 * res[0]
+    m0 = P.solve(np.zeros_like(sino), 3, 3, 3, 2, 0.0)
+    assert np.all(m0 == 0)
+
+
+def test_fp32_mirror_tracks_fp64(oracle_mod):
+    tilt, sino, _ = _case(snr=1.0)
+    m32 = oracle_mod.Plan(48, 48, tilt, 32).solve(sino, 3, 20, 3, 2, 0.0)
+    m64 = oracle_mod.Plan(48, 48, tilt, 64).solve(sino, 3, 20, 3, 2, 0.0)
+    assert rel(m32, m64) < 2e-5
+
+
+def test_reprojection_is_n_times_projection(oracle_mod):
+    """Re(A1D^H A_line m) at the min-|theta| tilt is N0 times that tilt's
+    projection (reProject_NFFT.cu:81-98)."""
+    N = 64
+    tilt, sino, b = _case(N=N, step=6)
+    P = oracle_mod.Plan(N, N, tilt, 64)
+    truth = phantom.render_2d(b, N, N)
+    rp = P.reproject(truth)
+    ref = phantom.project_2d(b, N, tilt[[P.iomit]])[0]
+    assert np.corrcoef(rp, ref)[0, 1] > 0.999
+    assert (rp @ ref) / (ref @ ref) == pytest.approx(N, rel=0.02)
