@@ -112,9 +112,14 @@ int upload_bins(icon_plan* p) {
 }
 
 void drop_graph(icon_plan* p) {
-  if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
-  p->graph_exec = nullptr;
-  p->graph_key = GraphKey{};
+  for (int i = 0; i < kGraphCache; i++) {
+    if (p->graph_exec[i]) hipGraphExecDestroy(p->graph_exec[i]);
+    p->graph_exec[i] = nullptr;
+    p->graph_key[i] = GraphKey{};
+    p->graph_used[i] = 0;
+  }
+  p->data_sino = nullptr;
+  p->data_S = 0;
 }
 
 // Eight grid columns per tile: two blocks fit in one CU's LDS at n0 = 1024
@@ -198,15 +203,17 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   const Geometry& g = p->g;
   const int iters = o.pre + o.icon + o.post;
   int rc;
-  // b = NFFT1D of every projection row; fw = w b; bnorm = sum w |b|^2
-  if ((rc = op_sino(p, S, sino, p->d_b, st))) return rc;
-  CombineArgs c0{};
-  c0.S = S; c0.M = g.M; c0.N0 = g.N0; c0.nz = g.nz; c0.mode = 2; c0.b = p->d_b; c0.w = p->d_w;
-  c0.fw = p->d_fw; c0.bnorm = p->d_bnorm; c0.nslot = p->d_nslot; c0.part = p->d_part;
-  c0.E = (int)p->bins.ent.size(); c0.ent_of_node = p->d_ent_of_node;
-  if ((rc = check_launch(launch_combine(c0, st), "combine_b"))) return rc;
-  // A^H W b
-  if ((rc = op_adjoint_ent(p, S, p->d_ahwb, st))) return rc;
+  if (!o.reuse_data) {
+    // b = NFFT1D of every projection row; fw = w b; bnorm = sum w |b|^2
+    if ((rc = op_sino(p, S, sino, p->d_b, st))) return rc;
+    CombineArgs c0{};
+    c0.S = S; c0.M = g.M; c0.N0 = g.N0; c0.nz = g.nz; c0.mode = 2; c0.b = p->d_b; c0.w = p->d_w;
+    c0.fw = p->d_fw; c0.bnorm = p->d_bnorm; c0.nslot = p->d_nslot; c0.part = p->d_part;
+    c0.E = (int)p->bins.ent.size(); c0.ent_of_node = p->d_ent_of_node;
+    if ((rc = check_launch(launch_combine(c0, st), "combine_b"))) return rc;
+    // A^H W b
+    if ((rc = op_adjoint_ent(p, S, p->d_ahwb, st))) return rc;
+  }
   // m = 0 (Iteration2.cu:33) or the warm start
   const size_t NN = (size_t)g.N0 * g.N1;
   if (m_init) rc = check_launch(launch_copy(m, m_init, (size_t)S * NN, st), "copy");
@@ -525,6 +532,8 @@ int icon_reconstruct(icon_plan* p, int S, const void* d_sino, void* d_m, const v
   const icon_solve_opts& o = *opts;
   if (o.pre < 0 || o.icon < 0 || o.post < 0) return fail(ICON_ERR_ARG, "negative iteration count");
   if (o.data_type != 1 && o.data_type != 2) return fail(ICON_ERR_ARG, "data_type must be 1 or 2");
+  if (o.reuse_data && !(p->data_sino == d_sino && p->data_S == S))
+    return fail(ICON_ERR_ARG, "reuse_data: the previous solve on this plan had other sinograms or batch");
   DevGuard dg(p->device);
   hipStream_t user = (hipStream_t)stream;
   HIPCHK(hipEventRecord(p->ev_in, user));
@@ -534,9 +543,17 @@ int icon_reconstruct(icon_plan* p, int S, const void* d_sino, void* d_m, const v
     GraphKey key;
     key.S = S; key.sino = d_sino; key.m = d_m; key.m_init = d_m_init; key.reproj = d_reproj;
     key.resid = d_resid; key.alpha = d_alpha; key.pre = o.pre; key.icon = o.icon; key.post = o.post;
-    key.data_type = o.data_type; key.thr = o.threshold;
-    if (!(p->graph_exec && p->graph_key == key)) {
-      drop_graph(p);
+    key.data_type = o.data_type; key.reuse = o.reuse_data ? 1 : 0; key.thr = o.threshold;
+    int slot = -1;
+    for (int i = 0; i < kGraphCache; i++)
+      if (p->graph_exec[i] && p->graph_key[i] == key) slot = i;
+    if (slot < 0) {
+      slot = 0;   // least recently used (empty slots have time 0)
+      for (int i = 1; i < kGraphCache; i++)
+        if (p->graph_used[i] < p->graph_used[slot]) slot = i;
+      if (p->graph_exec[slot]) hipGraphExecDestroy(p->graph_exec[slot]);
+      p->graph_exec[slot] = nullptr;
+      p->graph_key[slot] = GraphKey{};
       HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
       rc = enqueue_solve(p, S, (const float*)d_sino, (float*)d_m, (const float*)d_m_init, (float*)d_reproj,
                          (double*)d_resid, (double*)d_alpha, o, p->stream);
@@ -547,20 +564,23 @@ int icon_reconstruct(icon_plan* p, int S, const void* d_sino, void* d_m, const v
         return rc;
       }
       if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
-      e = hipGraphInstantiate(&p->graph_exec, graph, nullptr, nullptr, 0);
+      e = hipGraphInstantiate(&p->graph_exec[slot], graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (e != hipSuccess) {
-        p->graph_exec = nullptr;
+        p->graph_exec[slot] = nullptr;
         return fail(ICON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
       }
-      p->graph_key = key;
+      p->graph_key[slot] = key;
     }
-    HIPCHK(hipGraphLaunch(p->graph_exec, p->stream));
+    p->graph_used[slot] = ++p->graph_clock;
+    HIPCHK(hipGraphLaunch(p->graph_exec[slot], p->stream));
   } else {
     rc = enqueue_solve(p, S, (const float*)d_sino, (float*)d_m, (const float*)d_m_init, (float*)d_reproj,
                        (double*)d_resid, (double*)d_alpha, o, p->stream);
     if (rc) return rc;
   }
+  p->data_sino = d_sino;
+  p->data_S = S;
   HIPCHK(hipEventRecord(p->ev_out, p->stream));
   HIPCHK(hipStreamWaitEvent(user, p->ev_out, 0));
   return rc;
@@ -570,6 +590,7 @@ int icon_reconstruct_host(icon_plan* p, int S, const float* sino, float* m, cons
                           double* resid, double* alpha, const icon_solve_opts* opts) {
   if (!p || !sino || !m || !opts) return fail(ICON_ERR_ARG, "icon_reconstruct_host: null pointer");
   if (S < 1 || S > p->S_max) return fail(ICON_ERR_CAPACITY, "batch %d outside 1..%d", S, p->S_max);
+  if (opts->reuse_data) return fail(ICON_ERR_ARG, "icon_reconstruct_host: reuse_data needs device buffers");
   DevGuard dg(p->device);
   const Geometry& g = p->g;
   const int iters = opts->pre + opts->icon + opts->post;
@@ -614,6 +635,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   DevGuard dg(p->device);
   const Geometry& g = p->g;
   hipStream_t st = p->stream;
+  p->data_sino = nullptr;   // the timed kernels overwrite the solve's data buffers
   RowFwdArgs rf{};
   rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.cplx = 1; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
   rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;

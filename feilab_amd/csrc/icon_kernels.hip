@@ -155,9 +155,10 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   constexpr int LS = C + 1;
   constexpr int NG = NT / 16;
   extern __shared__ float2 tile[];   // NC * LS
-  const int nblk = a.n1 / C;
-  const int s = blockIdx.x / nblk;
-  const int blk = blockIdx.x % nblk;
+  // tile-major order: blocks in flight together work on the same column
+  // tile of different slices and share its bin records in L2
+  const int blk = blockIdx.x / a.S;
+  const int s = blockIdx.x % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int h0 = a.N0 >> 1;
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   const int lane = tid & 15;
   const int grp = tid >> 4;
   const int e1 = a.ent_off[blk + 1];
-  for (int e = a.ent_off[blk] + grp; e < e1; e += NG) {
+  auto interp = [&](int e) {
     const int4 en = a.ent[e];
     // lane q < 14: the node's x-tap q; every lane: the y-taps of the C columns
     const float px = lane < kTapsK ? a.ent_px[(size_t)e * kTapsK + lane] : 0.f;
@@ -202,7 +203,14 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
       ay += __shfl_xor(ay, o, 16);
     }
     if (lane == 0) a.part[((size_t)s * a.M + en.x) * kMaxSlotK + ((en.y >> 16) & 0xff)] = make_float2(ax, ay);
+  };
+  // two entries per group per iteration: their loads are in flight together
+  int e = a.ent_off[blk] + grp;
+  for (; e + NG < e1; e += 2 * NG) {
+    interp(e);
+    interp(e + NG);
   }
+  if (e < e1) interp(e);
 }
 
 // Wavefront-scope barrier for LDS data exchanged between the lanes of one wave.
@@ -222,9 +230,10 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
   constexpr int REGION = NC * LS * 2 / NW;       // floats of LDS staging per wave
   static_assert(64 * REC <= REGION, "staging chunk does not fit");
   extern __shared__ float2 tile[];   // NC * LS
-  const int nblk = a.n1 / C;
-  const int s = blockIdx.x / nblk;
-  const int blk = blockIdx.x % nblk;
+  // tile-major order: blocks in flight together work on the same column
+  // tile of different slices and share its bin records in L2
+  const int blk = blockIdx.x / a.S;
+  const int s = blockIdx.x % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -256,8 +265,21 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
     } else {
       lo1 = xo[NC - (kTapsK - 1) + xa]; hi1 = xo[NC]; lo2 = xo[0]; hi2 = xo[xa + 64];
     }
+    // this lane's 16-lane group owns rows [xg, xg+16): its candidates are the
+    // entries with first x-tap in [xg-13, xg+15], per run an index range
+    const int xg = xa + (lane & ~15);
+    int glo[2], ghi[2];
+    if (xg >= kTapsK - 1) {
+      glo[0] = ghi[0] = 0;
+      glo[1] = xo[xg - (kTapsK - 1)]; ghi[1] = xo[xg + 16];
+    } else {
+      glo[0] = xo[NC - (kTapsK - 1) + xg]; ghi[0] = xo[NC];
+      glo[1] = xo[0]; ghi[1] = xo[xg + 16];
+    }
     for (int run = 0; run < 2; run++) {
       const int lo = run ? lo2 : lo1, hi = run ? hi2 : hi1;
+      // the wrap run of the first wave is run 0; otherwise the single run is run 1
+      const int gr = (xa >= kTapsK - 1) ? 1 : run;
       for (int base = lo; base < hi; base += 64) {
         const int cnt = min(64, hi - base);
         wave_sync();
@@ -279,7 +301,10 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
           rec[kTapsK + C + 2] = __int_as_float(a.ent[e].z);
         }
         wave_sync();
-        for (int k = 0; k < cnt; k++) {
+        // the group's part of this chunk
+        const int k0 = max(glo[gr], base) - base;
+        const int k1 = min(ghi[gr], base + cnt) - base;
+        auto add = [&](int k) {
           const float* rec = stage + k * REC;
           const int tx = (x - __float_as_int(rec[kTapsK + C + 2])) & (NC - 1);
           if (tx < kTapsK) {
@@ -292,7 +317,13 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
               acc[r][c].y = __fmaf_rn(fy, phi, acc[r][c].y);
             }
           }
+        };
+        int k = k0;
+        for (; k + 1 < k1; k += 2) {
+          add(k);
+          add(k + 1);
         }
+        if (k < k1) add(k);
       }
     }
   }

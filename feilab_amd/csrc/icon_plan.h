@@ -59,14 +59,18 @@ struct GraphKey {
   void* reproj = nullptr;
   void* resid = nullptr;
   void* alpha = nullptr;
-  int pre = 0, icon = 0, post = 0, data_type = 0;
+  int pre = 0, icon = 0, post = 0, data_type = 0, reuse = 0;
   float thr = 0;
   bool operator==(const GraphKey& o) const {
     return S == o.S && sino == o.sino && m == o.m && m_init == o.m_init && reproj == o.reproj &&
            resid == o.resid && alpha == o.alpha && pre == o.pre && icon == o.icon &&
-           post == o.post && data_type == o.data_type && thr == o.thr;
+           post == o.post && data_type == o.data_type && reuse == o.reuse && thr == o.thr;
   }
 };
+
+// A few captured solves per plan (a warm-started solve alternates between two
+// or three schedules); least recently used is replaced.
+constexpr int kGraphCache = 4;
 
 }  // namespace icon
 
@@ -108,7 +112,12 @@ struct icon_plan {
   double* d_bnorm = nullptr;        // [S]
   double* d_alpha = nullptr;        // [S]
   int64_t device_bytes = 0;
-  // captured solve
-  hipGraphExec_t graph_exec = nullptr;
-  icon::GraphKey graph_key;
+  // captured solves
+  hipGraphExec_t graph_exec[icon::kGraphCache] = {};
+  icon::GraphKey graph_key[icon::kGraphCache];
+  int64_t graph_used[icon::kGraphCache] = {};
+  int64_t graph_clock = 0;
+  // sinograms whose b and A^H W b the per-slice buffers hold (reuse_data)
+  const void* data_sino = nullptr;
+  int data_S = 0;
 };

@@ -1,0 +1,544 @@
+// CDNA4 kernels of the learned warm start: a 2D U-Net on BF16 MFMA.
+//
+// Tensors are NHWC.  Forward activations are BF16 (stored as uint16), the
+// gradients FP32; weights keep an FP32 master copy and a BF16 compute copy in
+// the layout [tap][c_out][c_in] (c_in fastest), taps ky*3 + kx.
+//
+// The 3x3 (and 1x1) convolution is an implicit GEMM on
+// v_mfma_f32_16x16x32_bf16: M = pixels, N = output channels, K = taps x input
+// channels in groups of 32.  Lane l feeds A[pixel l%16][8 channels
+// 8*(l/16)...] and B[8 channels 8*(l/16)...][output channel l%16] as 16-byte
+// vectors straight from NHWC memory and receives D[pixel 4*(l/16)+i][channel
+// l%16].  The same kernel computes the input gradient with the spatially
+// flipped, channel-transposed weights; the weight gradient is a split-K GEMM
+// over pixels whose operands are transposed through LDS.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "unet_kernels.h"
+
+namespace unet {
+
+typedef __attribute__((ext_vector_type(4))) float f4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);   // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+union V8 {
+  uint4 u;
+  uint16_t h[8];
+  bf8 v;
+};
+
+// 8 consecutive channels of one pixel as a BF16 vector (zeros if !ok).
+template <typename T>
+__device__ __forceinline__ bf8 load8(const T* p, bool ok);
+
+template <>
+__device__ __forceinline__ bf8 load8<uint16_t>(const uint16_t* p, bool ok) {
+  V8 r;
+  r.u = ok ? *reinterpret_cast<const uint4*>(p) : make_uint4(0, 0, 0, 0);
+  return r.v;
+}
+
+template <>
+__device__ __forceinline__ bf8 load8<float>(const float* p, bool ok) {
+  V8 r;
+  if (ok) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0];
+    const float4 b = reinterpret_cast<const float4*>(p)[1];
+    r.h[0] = f2bf(a.x); r.h[1] = f2bf(a.y); r.h[2] = f2bf(a.z); r.h[3] = f2bf(a.w);
+    r.h[4] = f2bf(b.x); r.h[5] = f2bf(b.y); r.h[6] = f2bf(b.z); r.h[7] = f2bf(b.w);
+  } else {
+    r.u = make_uint4(0, 0, 0, 0);
+  }
+  return r.v;
+}
+
+__device__ __forceinline__ void store1(uint16_t* p, float v) { *p = f2bf(v); }
+__device__ __forceinline__ float load1(const uint16_t* p) { return bf2f(*p); }
+__device__ __forceinline__ float load1(const float* p) { return *p; }
+__device__ __forceinline__ void store1(float* p, float v) { *p = v; }
+
+// ----------------------------------------------------------------- conv
+// Block: 256 threads = 4 waves; tile 64 pixels x 64 output channels; wave w
+// computes pixels 32*(w&1).. x channels 32*(w>>1).. as 2 x 2 MFMA tiles.
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs<TX, TY> a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const long P = (long)a.N * a.H * a.W;
+  const long p_base = (long)blockIdx.x * 64 + 32 * (wave & 1);
+  const int co_base = blockIdx.y * 64 + 32 * (wave >> 1);
+  const int r16 = lane & 15, g = lane >> 4;
+  // the two pixels this lane feeds into A (one per 16-row sub-tile)
+  int pn[2], py[2], px[2];
+  bool pv[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const long p = p_base + 16 * i + r16;
+    pv[i] = p < P;
+    const long pp = pv[i] ? p : 0;
+    px[i] = (int)(pp % a.W);
+    py[i] = (int)((pp / a.W) % a.H);
+    pn[i] = (int)(pp / ((long)a.W * a.H));
+  }
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int half = a.taps == 9 ? 1 : 0;
+  for (int tap = 0; tap < a.taps; tap++) {
+    const int dy = tap / 3 * half - half, dx = tap % 3 * half - half;
+    const TX* src[2];
+    bool ok[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int yy = py[i] + dy, xx = px[i] + dx;
+      ok[i] = pv[i] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+      src[i] = a.x + (((long)pn[i] * a.H + (ok[i] ? yy : 0)) * a.W + (ok[i] ? xx : 0)) * a.ldx + a.cx0;
+    }
+    const uint16_t* wt = a.w + (long)tap * a.Cout * a.Cin;
+    for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
+      const int ci = ci0 + 8 * g;
+      const bool cok = ci < a.Cin;
+      bf8 A[2], B[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) A[i] = load8<TX>(src[i] + ci, ok[i] && cok);
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int co = co_base + 16 * j + r16;
+        B[j] = load8<uint16_t>(wt + (long)co * a.Cin + ci, cok && co < a.Cout);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // epilogue: bias, ReLU, gradient mask, store
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int co = co_base + 16 * j + r16;
+    if (co >= a.Cout) continue;
+    const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long p = p_base + 16 * i + 4 * g + r;
+        if (p >= P) continue;
+        float v = acc[i][j][r] + bias;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.mask && !(bf2f(a.mask[p * a.ldm + a.cm0 + co]) > 0.f)) v = 0.f;
+        TY* dst = a.y + p * a.ldy + a.cy0 + co;
+        if (a.accumulate) v += load1(dst);
+        store1(dst, v);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ weight grad
+// dW[tap][co][ci] += sum_p dY[p][co] * X[p + off(tap)][ci] over a pixel chunk.
+// Block 256 threads: output tile 64 co x 64 ci for one tap; K = pixels in
+// steps of 32 staged (transposed) in LDS; split-K over blockIdx.z with
+// float atomics into dW.
+__global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
+  __shared__ uint16_t sdy[64][40];   // [co][pixel], padded rows
+  __shared__ uint16_t sx[64][40];    // [ci][pixel]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tap = blockIdx.x % a.taps;
+  const int co_t = (blockIdx.x / a.taps) % ((a.Cout + 63) / 64);
+  const int ci_t = blockIdx.x / a.taps / ((a.Cout + 63) / 64);
+  const int half = a.taps == 9 ? 1 : 0;
+  const int dy = tap / 3 * half - half, dx = tap % 3 * half - half;
+  const long P = (long)a.N * a.H * a.W;
+  const long chunk = (P + gridDim.z - 1) / gridDim.z;
+  const long p0 = (long)blockIdx.z * chunk, p1 = std::min(P, p0 + chunk);
+  const int r16 = lane & 15, g = lane >> 4;
+  const int co_w = 32 * (wave >> 1), ci_w = 32 * (wave & 1);
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  // staging: thread t loads pixel (t & 31), 8 channels at 8*(t >> 5)
+  const int sp = threadIdx.x & 31, sc = 8 * (threadIdx.x >> 5);
+  for (long pb = p0; pb < p1; pb += 32) {
+    const long p = pb + sp;
+    {
+      // dY[p][co_t*64 + sc ..]
+      const int co = co_t * 64 + sc;
+      float v[8];
+      const bool ok = p < p1 && co < a.Cout;
+      if (ok) {
+        const float4 q0 = *reinterpret_cast<const float4*>(a.dy + p * a.lddy + a.cdy0 + co);
+        const float4 q1 = *reinterpret_cast<const float4*>(a.dy + p * a.lddy + a.cdy0 + co + 4);
+        v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w; v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) sdy[sc + q][sp] = ok ? f2bf(v[q]) : (uint16_t)0;
+      // X[p + off][ci_t*64 + sc ..]
+      const int ci = ci_t * 64 + sc;
+      bool okx = p < p1 && ci < a.Cin;
+      long src = 0;
+      if (okx) {
+        const int xx = (int)(p % a.W), yy = (int)((p / a.W) % a.H);
+        const long n = p / ((long)a.W * a.H);
+        const int y2 = yy + dy, x2 = xx + dx;
+        okx = y2 >= 0 && y2 < a.H && x2 >= 0 && x2 < a.W;
+        src = ((n * a.H + y2) * a.W + x2) * a.ldx + a.cx0 + ci;
+      }
+      V8 xv;
+      xv.u = okx ? *reinterpret_cast<const uint4*>(a.x + src) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 8; q++) sx[sc + q][sp] = xv.h[q];
+    }
+    __syncthreads();
+    // A[co][k = pixel] from sdy, B[k = pixel][ci] from sx
+    bf8 A[2], B[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      V8 t;
+      t.u = *reinterpret_cast<const uint4*>(&sdy[co_w + 16 * i + r16][8 * g]);
+      A[i] = t.v;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      V8 t;
+      t.u = *reinterpret_cast<const uint4*>(&sx[ci_w + 16 * j + r16][8 * g]);
+      B[j] = t.v;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D[co = 4g + r][ci = r16]
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int co = co_t * 64 + co_w + 16 * i + 4 * g + r;
+        const int ci = ci_t * 64 + ci_w + 16 * j + r16;
+        if (co < a.Cout && ci < a.Cin) atomicAdd(&a.dw[((long)tap * a.Cout + co) * a.Cin + ci], acc[i][j][r]);
+      }
+}
+
+// ------------------------------------------------------------ small ops
+
+// db[c] += sum_p dY[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db) {
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int part = threadIdx.x >> 6;
+  __shared__ float red[4][64];
+  float s = 0.f;
+  if (c < C)
+    for (long p = blockIdx.y * 4 + part; p < P; p += 4L * gridDim.y) s += dy[p * ld + c0 + c];
+  red[part][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (part == 0 && c < C) atomicAdd(&db[c], red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// 2x2 average pool (bf16 NHWC, channel sub-ranges)
+__global__ void k_pool_fwd(PoolArgs a) {
+  const long n = (long)a.N * (a.H / 2) * (a.W / 2) * a.C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % a.C);
+    const long q = i / a.C;
+    const int x = (int)(q % (a.W / 2)), y = (int)((q / (a.W / 2)) % (a.H / 2));
+    const long nn = q / ((long)(a.W / 2) * (a.H / 2));
+    const long b = ((nn * a.H + 2 * y) * a.W + 2 * x) * a.ldi + a.ci0 + c;
+    const float s = bf2f(a.in[b]) + bf2f(a.in[b + a.ldi]) + bf2f(a.in[b + (long)a.W * a.ldi]) +
+                    bf2f(a.in[b + (long)a.W * a.ldi + a.ldi]);
+    a.out[q * a.ldo + a.co0 + c] = f2bf(0.25f * s);
+  }
+}
+
+// gradient of the pool: dIn[2y+u][2x+v] (+)= dOut[y][x] / 4  (fp32)
+__global__ void k_pool_bwd(PoolGradArgs a) {
+  const long n = (long)a.N * a.H * a.W * a.C;   // H, W of the input
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % a.C);
+    const long q = i / a.C;
+    const int x = (int)(q % a.W), y = (int)((q / a.W) % a.H);
+    const long nn = q / ((long)a.W * a.H);
+    const float v = 0.25f * a.dout[((nn * (a.H / 2) + y / 2) * (a.W / 2) + x / 2) * a.ldo + a.co0 + c];
+    float* d = a.din + q * a.ldi + a.ci0 + c;
+    *d = a.accumulate ? *d + v : v;
+  }
+}
+
+// nearest 2x upsample (bf16): out[2y+u][2x+v] = in[y][x]
+__global__ void k_up_fwd(PoolArgs a) {
+  const long n = (long)a.N * (2 * a.H) * (2 * a.W) * a.C;   // H, W of the input
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % a.C);
+    const long q = i / a.C;
+    const int x = (int)(q % (2 * a.W)), y = (int)((q / (2 * a.W)) % (2 * a.H));
+    const long nn = q / ((long)(2 * a.W) * (2 * a.H));
+    a.out[q * a.ldo + a.co0 + c] = a.in[((nn * a.H + y / 2) * a.W + x / 2) * a.ldi + a.ci0 + c];
+  }
+}
+
+// gradient of the upsample: dIn[y][x] = sum of the 2x2 dOut block (fp32)
+__global__ void k_up_bwd(PoolGradArgs a) {
+  const long n = (long)a.N * a.H * a.W * a.C;   // H, W of the (small) input
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % a.C);
+    const long q = i / a.C;
+    const int x = (int)(q % a.W), y = (int)((q / a.W) % a.H);
+    const long nn = q / ((long)a.W * a.H);
+    const long b = ((nn * 2 * a.H + 2 * y) * 2 * a.W + 2 * x) * a.ldo + a.co0 + c;
+    const float v = a.dout[b] + a.dout[b + a.ldo] + a.dout[b + 2L * a.W * a.ldo] + a.dout[b + 2L * a.W * a.ldo + a.ldo];
+    float* d = a.din + q * a.ldi + a.ci0 + c;
+    *d = a.accumulate ? *d + v : v;
+  }
+}
+
+// dIn *= (act > 0) in place (fp32 gradient, bf16 activation)
+__global__ void k_relu_mask(float* g, int ldg, int cg0, const uint16_t* act, int lda, int ca0, long P, int C) {
+  const long n = P * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / C;
+    const int c = (int)(i % C);
+    if (!(bf2f(act[p * lda + ca0 + c]) > 0.f)) g[p * ldg + cg0 + c] = 0.f;
+  }
+}
+
+// 1x1 head: out[p] = sum_c x[p][c] w[c] + b  (fp32 out)
+__global__ void k_head_fwd(const uint16_t* x, int ldx, const float* w, const float* b, int C, long P, float* out) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+    float s = b[0];
+    for (int c = 0; c < C; c++) s = fmaf(bf2f(x[p * ldx + c]), w[c], s);
+    out[p] = s;
+  }
+}
+
+// head backward: dX[p][c] = dout[p] w[c] (masked by ReLU of x); dw, db reduced
+__global__ __launch_bounds__(256) void k_head_bwd(const uint16_t* x, int ldx, const float* w, int C, long P,
+                                                 const float* dout, float* dx, int lddx, float* dw, float* db) {
+  __shared__ float red[256];
+  float gw[64];
+  for (int c = 0; c < C; c++) gw[c] = 0.f;
+  float gb = 0.f;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+    const float d = dout[p];
+    gb += d;
+    for (int c = 0; c < C; c++) {
+      const float xv = bf2f(x[p * ldx + c]);
+      gw[c] = fmaf(d, xv, gw[c]);
+      dx[p * lddx + c] = xv > 0.f ? d * w[c] : 0.f;
+    }
+  }
+  for (int c = 0; c <= C; c++) {
+    red[threadIdx.x] = c < C ? gw[c] : gb;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(c < C ? &dw[c] : db, red[0]);
+    __syncthreads();
+  }
+}
+
+// dgrad weights: wt[tap'][ci][co] = w[8 - tap'][co][ci]  (bf16)
+__global__ void k_flip_weights(const uint16_t* w, uint16_t* wt, int taps, int Cout, int Cin) {
+  const long n = (long)taps * Cout * Cin;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int co = (int)(i % Cout);
+    const int ci = (int)((i / Cout) % Cin);
+    const int tp = (int)(i / ((long)Cout * Cin));
+    wt[i] = w[((long)(taps - 1 - tp) * Cout + co) * Cin + ci];
+  }
+}
+
+// Adam step on fp32 master weights, refreshes the bf16 copy.
+__global__ void k_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, float gscale) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    if (w16) w16[i] = f2bf(wi);
+  }
+}
+
+__global__ void k_f32_to_bf16(const float* in, uint16_t* out, long n, int ldo, int co0, int C) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / C;
+    const int c = (int)(i % C);
+    out[p * ldo + co0 + c] = f2bf(in[i]);
+  }
+}
+
+// MSE loss on scaled residual targets: d = pred - target; loss partial sums
+__global__ __launch_bounds__(256) void k_mse(const float* pred, const float* target, long P, float* dpred,
+                                            double* loss, float scale) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+    const float d = pred[p] - target[p];
+    s += (double)d * d;
+    dpred[p] = scale * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// Per-image scale of the warm start: s = RMS of the image (1 for a zero image),
+// accumulated in FP64.  One block per image.
+__global__ __launch_bounds__(256) void k_image_rms(const float* in, long P, float* scale) {
+  __shared__ double red[256];
+  const float* x = in + (long)blockIdx.x * P;
+  double s = 0.0;
+  for (long p = threadIdx.x; p < P; p += blockDim.x) s += (double)x[p] * x[p];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double r = sqrt(red[0] / (double)P);
+    scale[blockIdx.x] = r > 0.0 ? (float)r : 1.f;
+  }
+}
+
+// out[p][0] = bf16(in[p] / scale[image]) into an 8-channel NHWC buffer
+__global__ void k_to_bf16_scaled(const float* in, const float* scale, long P, long n, uint16_t* out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i * 8] = f2bf(in[i] / scale[i / P]);
+}
+
+// out = m + scale[image] * y
+__global__ void k_residual_add(const float* m, const float* y, const float* scale, long P, long n, float* out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = fmaf(scale[i / P], y[i], m[i]);
+}
+
+__global__ void k_zero(float* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+// ================================================================ launchers
+
+static int blocks_for(long n, int per = 256) { return (int)std::min<long>((n + per - 1) / per, 65536); }
+
+template <typename TX, typename TY>
+hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
+  const long P = (long)a.N * a.H * a.W;
+  dim3 grid((unsigned)((P + 63) / 64), (unsigned)((a.Cout + 63) / 64));
+  hipLaunchKernelGGL((k_conv<TX, TY>), grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+template hipError_t launch_conv<uint16_t, uint16_t>(const ConvArgs<uint16_t, uint16_t>&, hipStream_t);
+template hipError_t launch_conv<float, float>(const ConvArgs<float, float>&, hipStream_t);
+
+hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st) {
+  const int nco = (a.Cout + 63) / 64, nci = (a.Cin + 63) / 64;
+  hipLaunchKernelGGL(k_conv_wgrad, dim3(a.taps * nco * nci, 1, splits), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db, hipStream_t st) {
+  hipLaunchKernelGGL(k_bias_grad, dim3((C + 63) / 64, 256), dim3(256), 0, st, dy, ld, c0, P, C, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_fwd(const PoolArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_pool_bwd(const PoolGradArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_up_fwd(const PoolArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_up_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C * 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_up_bwd(const PoolGradArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_up_bwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_relu_mask(float* g, int ldg, int cg0, const uint16_t* act, int lda, int ca0, long P, int C,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_relu_mask, dim3(blocks_for(P * C)), dim3(256), 0, st, g, ldg, cg0, act, lda, ca0, P, C);
+  return hipGetLastError();
+}
+hipError_t launch_head_fwd(const uint16_t* x, int ldx, const float* w, const float* b, int C, long P, float* out,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(P)), dim3(256), 0, st, x, ldx, w, b, C, P, out);
+  return hipGetLastError();
+}
+hipError_t launch_head_bwd(const uint16_t* x, int ldx, const float* w, int C, long P, const float* dout, float* dx,
+                           int lddx, float* dw, float* db, hipStream_t st) {
+  hipLaunchKernelGGL(k_head_bwd, dim3(256), dim3(256), 0, st, x, ldx, w, C, P, dout, dx, lddx, dw, db);
+  return hipGetLastError();
+}
+hipError_t launch_flip_weights(const uint16_t* w, uint16_t* wt, int taps, int Cout, int Cin, hipStream_t st) {
+  hipLaunchKernelGGL(k_flip_weights, dim3(blocks_for((long)taps * Cout * Cin)), dim3(256), 0, st, w, wt, taps,
+                     Cout, Cin);
+  return hipGetLastError();
+}
+hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, float gscale, hipStream_t st) {
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, st, w, w16, g, m, v, n, lr, b1, b2, eps, c1, c2,
+                     gscale);
+  return hipGetLastError();
+}
+hipError_t launch_f32_to_bf16(const float* in, uint16_t* out, long n, int ldo, int co0, int C, hipStream_t st) {
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks_for(n)), dim3(256), 0, st, in, out, n, ldo, co0, C);
+  return hipGetLastError();
+}
+hipError_t launch_mse(const float* pred, const float* target, long P, float* dpred, double* loss, float scale,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_mse, dim3(blocks_for(P, 1024)), dim3(256), 0, st, pred, target, P, dpred, loss, scale);
+  return hipGetLastError();
+}
+hipError_t launch_image_rms(const float* in, long P, int B, float* scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_image_rms, dim3(B), dim3(256), 0, st, in, P, scale);
+  return hipGetLastError();
+}
+hipError_t launch_to_bf16_scaled(const float* in, const float* scale, long P, int B, uint16_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf16_scaled, dim3(blocks_for(P * B)), dim3(256), 0, st, in, scale, P, P * B, out);
+  return hipGetLastError();
+}
+hipError_t launch_residual_add(const float* m, const float* y, const float* scale, long P, int B, float* out,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_residual_add, dim3(blocks_for(P * B)), dim3(256), 0, st, m, y, scale, P, P * B, out);
+  return hipGetLastError();
+}
+hipError_t launch_zero32(float* p, long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, dim3(blocks_for(n)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace unet

@@ -43,7 +43,8 @@ class PlanInfo(ctypes.Structure):
 class SolveOptions(ctypes.Structure):
     """The reference's before[] / betain[] (ICONGPU.cu:380-387)."""
     _fields_ = [("pre", ctypes.c_int32), ("icon", ctypes.c_int32), ("post", ctypes.c_int32),
-                ("data_type", ctypes.c_int32), ("threshold", ctypes.c_float), ("use_graph", ctypes.c_int32)]
+                ("data_type", ctypes.c_int32), ("threshold", ctypes.c_float), ("use_graph", ctypes.c_int32),
+                ("reuse_data", ctypes.c_int32)]
 
 
 def lib_path():

@@ -64,6 +64,10 @@ typedef struct {
   int32_t data_type;  /* 1: clamp m <= thr, 2: clamp m >= thr   (-d)        */
   float threshold;    /*                                        (-thr)      */
   int32_t use_graph;  /* 1: replay the iteration loop as a HIP graph       */
+  int32_t reuse_data; /* 1: keep b and A^H W b from the previous call on this
+                         plan, which must have had the same d_sino and S (the
+                         caller keeps the sinograms unchanged); a warm-started
+                         continuation then skips the data set-up */
 } icon_solve_opts;
 
 /* ---- errors / devices / memory ---------------------------------------- */

@@ -37,7 +37,7 @@
 //   reProject_NFFT.cu:81-98         reprojection at the min-|theta| tilt
 //
 // Parity status of each function is stated in its comment; the pinning tests
-// live in tests/test_oracle_*.py.
+// live in tests/test_oracle.py.
 
 #include <cmath>
 #include <cstdio>

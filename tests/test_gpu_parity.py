@@ -174,6 +174,33 @@ def test_graph_replay_matches_eager(icon):
         plan.solve(s0, 1, 1, 1, 2, 0.0)  # eager work between replays
 
 
+def test_split_solve_with_reuse_data_matches_full(icon):
+    """A solve stopped after k iterations and continued from its iterate, with
+    the data set-up reused, equals the uninterrupted solve (the iteration
+    state is m alone); three captured schedules alternate in the graph cache."""
+    _print("split solve reuse_data")
+    from feilab_amd.icon import DeviceArray, SolveOptions
+    tilt, s0 = _small_case(29)
+    _, s1 = _small_case(30)
+    N = s0.shape[1]
+    plan = icon.Plan(N, tilt, max_slices=2)
+    sino = np.stack([s0, s1])
+    full = plan.solve(sino, 4, 12, 4, 2, 0.0)
+    d_s = DeviceArray.from_host(sino)
+    d_a = DeviceArray((2, N, N), np.float32)
+    d_b = DeviceArray((2, N, N), np.float32)
+    first = SolveOptions(3, 0, 0, 2, 0.0, 1, 0)
+    rest = SolveOptions(1, 12, 4, 2, 0.0, 1, 1)
+    for rep in range(2):
+        plan.reconstruct_dev(2, d_s, d_a, first)
+        plan.reconstruct_dev(2, d_s, d_b, rest, d_m_init=d_a)
+        icon.lib().icon_device_synchronize()
+        assert rel(d_b.numpy(), full) < 1e-6, rep
+        plan.solve(sino, 1, 1, 1, 2, 0.0)   # other work: the next reuse must be refused
+        with pytest.raises(icon.IconError):
+            plan.reconstruct_dev(2, d_s, d_b, rest, d_m_init=d_a)
+
+
 def test_warm_start_continues(icon, oracle_mod):
     """A warm start m_init replaces the zero start (Iteration2.cu:33)."""
     tilt, s0 = _small_case(41)
