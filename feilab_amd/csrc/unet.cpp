@@ -183,11 +183,10 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   const long P = (long)B * H * W;
   auto& cv = n->convs;
   int rc;
-#define R(x) if ((rc = (x))) return rc
-  if (in_scale)
-    R(launch_to_bf16_scaled(d_in, in_scale, (long)H * W, B, n->x8.p, st) == hipSuccess ? 0 : ufail(2, "to_bf16"));
-  else
-    R(launch_f32_to_bf16(d_in, n->x8.p, P, 8, 0, 1, st) == hipSuccess ? 0 : ufail(2, "to_bf16"));
+#define R(x) do { if ((rc = (x))) return rc; } while (0)
+  const hipError_t ce = in_scale ? launch_to_bf16_scaled(d_in, in_scale, (long)H * W, B, n->x8.p, st)
+                                 : launch_f32_to_bf16(d_in, n->x8.p, P, 8, 0, 1, st);
+  if (ce != hipSuccess) return ufail(2, "input conversion: %s", hipGetErrorString(ce));
   R(conv(n, cv[E1A], n->x8.p, 8, 0, n->a1.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
   R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, 96, 64, B, H, W, true, nullptr, 0, 0, false, false, st));
   R(pool(n->cat1, 64, 32, n->p1, B, H, W, st));
@@ -219,8 +218,8 @@ int backward(icon_unet* n, int B, const float* d_dout, hipStream_t st) {
   const long P = (long)B * H * W;
   auto& cv = n->convs;
   int rc;
-#define R(x) if ((rc = (x))) return rc
-#define HIPR(x) if ((x) != hipSuccess) return ufail(2, "%s failed", #x)
+#define R(x) do { if ((rc = (x))) return rc; } while (0)
+#define HIPR(x) do { if ((x) != hipSuccess) return ufail(2, "%s failed", #x); } while (0)
   HIPR(launch_head_bwd(n->d1b.p, 32, n->params + n->head_w, 32, P, d_dout, n->g_d1b.p, 32, n->grads + n->head_w,
                        n->grads + n->head_b, st));
   // d1b, d1a

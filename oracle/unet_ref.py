@@ -2,10 +2,10 @@
 
 Float64 torch-CPU restatement of the network in feilab_amd/csrc/unet.cpp,
 reading the same flat parameter vector.  With bf16=True it rounds where the
-device rounds (input, 3x3 weights, every convolution/pool output) so that the
-remaining differences are accumulation order and the device's BF16 rounding
-of gradient operands.  Used only by tests to check the device forward pass
-and gradients.
+device rounds (input, 3x3 weights, every convolution/pool output, and the
+output gradient of every 3x3 convolution before its input- and weight-gradient
+products) so that the remaining differences are accumulation order and FP32
+rounding.  Used only by tests to check the device forward pass and gradients.
 """
 import numpy as np
 import torch
@@ -24,6 +24,25 @@ def _bf16(t):
     # round to bfloat16 (nearest even), keep float64; gradient passes straight through
     r = t.to(torch.bfloat16).to(torch.float64)
     return t + (r - t).detach()
+
+
+class _ConvBF16Grad(torch.autograd.Function):
+    """3x3 convolution whose backward rounds the output gradient to BF16
+    before the input- and weight-gradient products, as the device's MFMA
+    kernels do; the bias gradient sums the unrounded gradient."""
+
+    @staticmethod
+    def forward(ctx, x, w, b):
+        ctx.save_for_backward(x, w)
+        return F.conv2d(x, w, b, padding=1)
+
+    @staticmethod
+    def backward(ctx, g):
+        x, w = ctx.saved_tensors
+        gr = g.to(torch.bfloat16).to(torch.float64)
+        gx = torch.nn.grad.conv2d_input(x.shape, w, gr, padding=1)
+        gw = torch.nn.grad.conv2d_weight(x, w.shape, gr, padding=1)
+        return gx, gw, g.sum((0, 2, 3))
 
 
 def unpack(params, requires_grad=False):
@@ -49,7 +68,8 @@ def forward(params, x, bf16=True, requires_grad=False):
 
     def conv(t, i):
         w, b = L[i]
-        return rnd(F.relu(F.conv2d(t, rnd(w), b, padding=1)))
+        y = _ConvBF16Grad.apply(t, rnd(w), b) if bf16 else F.conv2d(t, w, b, padding=1)
+        return rnd(F.relu(y))
 
     pool = lambda t: rnd(F.avg_pool2d(t, 2))
     up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")
@@ -69,4 +89,4 @@ def gradients(params, x, target, bf16=True):
     t = torch.tensor(np.asarray(target, np.float64))
     loss = ((out - t) ** 2).mean()
     loss.backward()
-    return float(loss), p.grad.numpy()
+    return float(loss.detach()), p.grad.numpy()

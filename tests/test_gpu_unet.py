@@ -6,8 +6,11 @@ activation).  Tolerances:
 
 * forward: relative L2 <= 1e-2 (accumulation order flips an occasional BF16
   rounding; a wrong operand layout or tap gives O(1) errors);
-* gradients: per parameter block relative L2 <= 5e-2 and cosine >= 0.998
-  (the device also rounds the gradient operands of its MFMA products to BF16).
+* gradients: cosine >= 0.999 overall, and per parameter block a relative L2
+  distance to the BF16 oracle of at most twice the oracle's own BF16-vs-FP64
+  distance for that block (1e-2 at least): two BF16 computations that round
+  in a different order differ by about the rounding effect itself, while a
+  wrong layout, tap or mask gives O(1).
 """
 import numpy as np
 import pytest
@@ -77,17 +80,22 @@ def test_unet_gradients_vs_oracle(icon, shape):
     t = (0.5 * x + 0.1 * rng.standard_normal(shape)).astype(np.float32)
     loss, g = net.gradients(x, t)
     rloss, rg = unet_ref.gradients(p, x, t, bf16=True)
+    _, rg64 = unet_ref.gradients(p, x, t, bf16=False)
     assert abs(loss - rloss) <= 1e-2 * abs(rloss)
+    gd_all = g.astype(np.float64)
+    assert float(gd_all @ rg / (np.linalg.norm(gd_all) * np.linalg.norm(rg))) > 0.999
     worst = []
     for name, a, b in _blocks():
-        gd, gr = g[a:b].astype(np.float64), rg[a:b]
+        gd, gr, g64 = g[a:b].astype(np.float64), rg[a:b], rg64[a:b]
         if np.linalg.norm(gr) == 0:
             assert np.linalg.norm(gd) == 0, name
             continue
-        cos = float(gd @ gr / (np.linalg.norm(gd) * np.linalg.norm(gr)))
-        worst.append((rel(gd, gr), cos, name))
-        assert rel(gd, gr) < 5e-2 and cos > 0.998, (name, rel(gd, gr), cos)
-    _print("worst block", max(worst))
+        # BF16 rounding alone moves this block by `noise` (oracle BF16 vs FP64);
+        # two BF16 computations with different rounding order differ by about as much
+        noise = rel(gr, g64)
+        worst.append((rel(gd, gr) / max(noise, 5e-3), rel(gd, gr), noise, name))
+        assert rel(gd, gr) <= max(2.0 * noise, 1e-2), (name, rel(gd, gr), noise)
+    _print("worst block (ratio, rel, bf16 noise, name)", max(worst))
     net.close()
 
 

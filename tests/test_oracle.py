@@ -203,3 +203,30 @@ def test_reprojection_is_n_times_projection(oracle_mod):
     ref = phantom.project_2d(b, N, tilt[[P.iomit]])[0]
     assert np.corrcoef(rp, ref)[0, 1] > 0.999
     assert (rp @ ref) / (ref @ ref) == pytest.approx(N, rel=0.02)
+
+
+def test_unet_oracle_gradient_matches_finite_differences():
+    """The U-Net oracle's backward (float64, no BF16 rounding) against central
+    differences of its forward along random directions; with BF16 rounding the
+    gradient stays within the rounding-level distance of the exact one."""
+    from oracle import unet_ref
+    rng = np.random.default_rng(0)
+    n = unet_ref.param_count()
+    p = rng.standard_normal(n) * 0.05
+    x = rng.standard_normal((1, 16, 16))
+    t = 0.5 * x
+    loss, g = unet_ref.gradients(p, x, t, bf16=False)
+
+    def L(q):
+        out, _ = unet_ref.forward(q, x, bf16=False)
+        return float(((out.detach().numpy() - t) ** 2).mean())
+
+    for k in range(3):
+        d = rng.standard_normal(n)
+        d /= np.linalg.norm(d)
+        h = 1e-4
+        fd = (L(p + h * d) - L(p - h * d)) / (2 * h)
+        assert abs(fd - g @ d) <= 1e-6 + 1e-5 * abs(fd), (k, fd, g @ d)
+    _, gb = unet_ref.gradients(p, x, t, bf16=True)
+    assert np.linalg.norm(gb - g) / np.linalg.norm(g) < 0.1
+    assert float(gb @ g / (np.linalg.norm(gb) * np.linalg.norm(g))) > 0.99
