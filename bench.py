@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
-"""Throughput of the MI355X ICON core on the batched 512^2 tilt-series workload.
+"""Throughput of the MI355X ICON core on the batched 512^2 tilt-series workload:
+slices/s to a fixed data residual, learned start against plain ICON.
 
-One step = one batched ICON solve of every slice of the synthetic tilt series
-(256 slices of 512 x 512, 41 tilts +-60 deg in 3 deg steps, 200 iterations
-split 20 INFR / 160 ICON / 20 INFR, threshold m >= 0) -- the whole solve
-replayed as one HIP graph with the sinograms already resident in HBM.  The
-data residual R(m) = sqrt(sum w|Am-b|^2 / sum w|b|^2) (FP64) of every slice is
-produced inside the timed region.
+Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
+41 tilts +-60 deg in 3 deg steps, SNR 0.05), sinograms resident in HBM.
 
-Prints one JSON line with the throughput (slices/s), the roofline of the
-dominant kernel (algorithmic bytes / HIP-event-timed launch duration against
-8 TB/s HBM) and the CPU baseline (the FP32-mirror oracle on the host cores on a
-bounded sample of the same workload).
+* Plain leg: one step = the batched plain ICON solve (200 iterations split
+  20 INFR / 160 ICON / 20 INFR, threshold m >= 0) replayed as one HIP graph.
+  Its per-slice data residual R(m) = sqrt(sum w|Am-b|^2 / sum w|b|^2) (FP64)
+  is the target, its reprojection FRC at the min-|theta| tilt the reference.
+* Learned leg: a U-Net trained on the device (its own BF16 MFMA kernels) on
+  (10-iteration, 200-iteration) pairs of a separate phantom; one step = 10
+  plain iterations, the U-Net warm start, and k descent iterations, with k the
+  smallest continuation that brings every slice to R <= its plain target with
+  the reprojection FRC (rings 1..N/4) within 0.005 of plain ICON's.  Slices
+  that miss the target are re-solved by plain ICON (counted as fallbacks).
+
+Prints one JSON line: value = learned-start slices/s, plus the plain leg,
+the roofline of the dominant ICON kernel (algorithmic bytes / HIP-event-timed
+launch duration against 8 TB/s HBM) and the CPU baseline (the FP32-mirror
+oracle on the host cores on a bounded sample of the plain workload).
 """
 import argparse
 import json
@@ -76,8 +84,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--iters", default="20,160,20", help="pre,icon,post")
+    ap.add_argument("--start", type=int, default=10, help="plain iterations before the U-Net")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-slices", type=int, default=256)
+    ap.add_argument("--unet-batch", type=int, default=16)
+    ap.add_argument("--max-cont", type=int, default=40, help="continuation iterations tried")
+    ap.add_argument("--frc-tol", type=float, default=0.005)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=16)
-    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
@@ -89,7 +104,7 @@ def main():
         sys.exit(2)
 
     import feilab_amd
-    from feilab_amd import phantom
+    from feilab_amd import phantom, warmstart
     from feilab_amd.icon import DeviceArray, SolveOptions
 
     pre, icon_it, post = [int(v) for v in args.iters.split(",")]
@@ -103,30 +118,35 @@ def main():
     plan = feilab_amd.Plan(cfg["N"], cfg["tilt"], max_slices=S, device=0)
     if args.col_block:
         plan.set_col_block(args.col_block)
+    iomit = plan.info.iomit
+    band = (1, plan.N0 // 4)
     d_sino = DeviceArray.from_host(cfg["sino"], np.float32)
     d_m = DeviceArray((S, plan.N0, plan.N1), np.float32)
     d_res = DeviceArray((S, iters + 1), np.float64)
-    opts = SolveOptions(pre, icon_it, post, 2, 0.0, 1)
+    d_rp = DeviceArray((S, plan.N0), np.float32)
+    opts = SolveOptions(pre, icon_it, post, 2, 0.0, 1, 0)
     L = feilab_amd.lib()
 
-    def step():
-        plan.reconstruct_dev(S, d_sino, d_m, opts, d_resid=d_res)
-
-    for w in range(args.warmup):
-        step()
+    def timed(step, what):
+        for w in range(args.warmup):
+            step()
+            L.icon_device_synchronize()
+            log("%s warmup step %d done" % (what, w))
         L.icon_device_synchronize()
-        log("warmup step %d done" % w)
-    L.icon_device_synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-    L.icon_device_synchronize()
-    dt = time.perf_counter() - t0
-    log("timed %d steps in %.3f s" % (args.steps, dt))
-    ms_step = dt / args.steps * 1e3
-    value = S * args.steps / dt
-    resid = d_res.numpy()
-    final_res = resid[:, -1]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step()
+        L.icon_device_synchronize()
+        dt = time.perf_counter() - t0
+        log("%s: timed %d steps in %.3f s" % (what, args.steps, dt))
+        return dt
+
+    # ---- plain leg
+    dt_plain = timed(lambda: plan.reconstruct_dev(S, d_sino, d_m, opts, d_resid=d_res, d_reproj=d_rp), "plain")
+    plain_value = S * args.steps / dt_plain
+    R_plain = d_res.numpy()[:, -1]
+    frc_plain_curve, frc_plain = warmstart.reprojection_frc(d_rp.numpy(), cfg["sino"], iomit, band)
+    log("plain: %.2f slices/s, R mean %.5f, FRC %.4f" % (plain_value, R_plain.mean(), frc_plain))
 
     # per-kernel launch durations (HIP events on the plan's stream)
     kb = algorithmic_bytes(plan)
@@ -144,11 +164,74 @@ def main():
     roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
                     algo_bytes="%d per slice x %d slices per launch" % (kb[dom], S))
+
+    # ---- learned leg
+    learned = None
+    value, ms_step = plain_value, dt_plain / args.steps * 1e3
+    if not args.no_learned:
+        log("training the warm start: %d slices of a separate phantom, %d steps" % (args.train_slices, args.train_steps))
+        t0 = time.perf_counter()
+        tr = phantom.config2(nslices=args.train_slices, seed=2001)
+        params, losses = warmstart.train_warm_start(plan, tr["sino"], args.start, (pre, icon_it, post),
+                                                    steps=args.train_steps, log=log)
+        train_s = time.perf_counter() - t0
+        solver = warmstart.LearnedSolver(plan, params, S, start=args.start, batch=args.unet_batch)
+        d_ml = DeviceArray((S, plan.N0, plan.N1), np.float32)
+
+        def check(k, use_net):
+            d_r = DeviceArray((S, k + 1), np.float64)
+            solver.run(S, d_sino, d_ml, k, d_resid=d_r, d_reproj=d_rp, use_net=use_net)
+            L.icon_device_synchronize()
+            R = d_r.numpy()[:, -1]
+            _, f = warmstart.reprojection_frc(d_rp.numpy(), cfg["sino"], iomit, band)
+            ok = bool((R <= R_plain).all() and abs(f - frc_plain) <= args.frc_tol)
+            return ok, R, f
+
+        k_star, res = None, None
+        for k in range(1, args.max_cont + 1):
+            ok, R, f = check(k, True)
+            if ok:
+                k_star, res = k, (R, f)
+                break
+        nonet = []
+        for k in sorted({1, 5, 10, 20, args.max_cont} | ({k_star} if k_star else set())):
+            ok, R, f = check(k, False)
+            nonet.append(dict(cont=k, meets_target=ok, resid_max_over_target=round(float((R / R_plain).max()), 4),
+                              frc=round(f, 4)))
+        log("no-network continuation:", json.dumps(nonet))
+        if k_star is None:
+            log("learned start missed the target within %d continuation iterations: plain ICON is the result" % args.max_cont)
+            learned = dict(k=None, fallback="all slices (plain ICON)", train_seconds=round(train_s, 1), no_network=nonet)
+        else:
+            dt_l = timed(lambda: solver.run(S, d_sino, d_ml, k_star), "learned")
+            value = S * args.steps / dt_l
+            ms_step = dt_l / args.steps * 1e3
+            # verify the timed result and count slices that would fall back to plain ICON
+            d_r = DeviceArray((S, k_star + 1), np.float64)
+            solver.run(S, d_sino, d_ml, k_star, d_resid=d_r, d_reproj=d_rp)
+            L.icon_device_synchronize()
+            R = d_r.numpy()[:, -1]
+            _, f = warmstart.reprojection_frc(d_rp.numpy(), cfg["sino"], iomit, band)
+            fallback = int((R > R_plain).sum())
+            learned = dict(start_iterations=args.start, continuation_iterations=k_star,
+                           slices_per_s=round(value, 3), speedup_vs_plain=round(value / plain_value, 3),
+                           resid_mean=float(R.mean()), resid_max_over_target=round(float((R / R_plain).max()), 5),
+                           frc=round(f, 5), frc_plain=round(frc_plain, 5), frc_band=list(band),
+                           fallback_slices=fallback, train_steps=args.train_steps,
+                           train_loss_last=round(float(np.mean(losses[-50:])), 4), train_seconds=round(train_s, 1),
+                           unet_batch=args.unet_batch, no_network=nonet)
+            log("learned:", json.dumps(learned))
+        solver.close()
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: %d slices x %d iterations" % (args.cpu_slices, args.cpu_iters))
         cpu = cpu_baseline(cfg, args.cpu_slices, args.cpu_iters, iters, min(args.cpu_threads, os.cpu_count() or 1))
         log("cpu baseline done")
+    workload = ("learned-start ICON vs plain ICON: %d slices of 512x512, 41 tilts +-60 deg step 3, SNR 0.05; "
+                "target = per-slice data residual of plain ICON (%d iterations %d,%d,%d, m >= 0) with reprojection "
+                "FRC within %.3f; sinograms resident in HBM, solves replayed as HIP graphs"
+                % (S, iters, pre, icon_it, post, args.frc_tol))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -161,14 +244,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic",
+        "data": "synthetic (seeded blob phantoms, analytic projections, Gaussian noise)",
         "config": {
-            "workload": "batched plain ICON: %d slices of 512x512, 41 tilts +-60 deg step 3, %d iterations "
-                        "(%d,%d,%d), m >= 0 threshold, all slices per launch, whole solve as one HIP graph"
-                        % (S, iters, pre, icon_it, post),
-            "slices": S, "N": 512, "tilts": 41, "iterations": iters, "start": "plain (zeros)",
-            "fixed_residual": "R after %d plain iterations" % iters,
-            "final_residual_mean": float(final_res.mean()), "final_residual_max": float(final_res.max()),
+            "workload": workload,
+            "slices": S, "N": 512, "tilts": 41, "plain_iterations": iters,
+            "plain_slices_per_s": round(plain_value, 3),
+            "plain_resid_mean": float(R_plain.mean()), "plain_resid_max": float(R_plain.max()),
+            "plain_frc": round(frc_plain, 5),
+            "learned": learned,
             "col_block": int(plan.info.col_block),
             "ms_per_iteration_kernels": round(per_iter_ms, 3),
         },

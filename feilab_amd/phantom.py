@@ -106,3 +106,25 @@ def config2(nslices=256, seed=1002, N=512, depth=512, nblobs=400, snr=0.05):
     clean = project_3d(b, N, np.arange(nslices), tilt)
     return dict(N=N, tilt=tilt, blobs=b, clean=clean.astype(np.float32), sino=add_noise(clean, snr, rng),
                 pre=20, icon=160, post=20)
+
+
+def render_slices_3d(b, N0, N1, slices):
+    """Ground truth of a 3D blob set on the reconstruction grid of each slice:
+    [len(slices)][N0 (x)][N1 (z)], the coordinates of project_3d."""
+    slices = np.asarray(slices)
+    out = np.zeros((len(slices), N0, N1))
+    for x, z, y, s, a in zip(b["x"], b["z"], b["y"], b["s"], b["a"]):
+        wy = a * np.exp(-(slices - y) ** 2 / (2 * s * s))
+        keep = np.nonzero(wy > 1e-7)[0]
+        if keep.size == 0:
+            continue
+        r = int(np.ceil(5 * s))
+        cx, cz = x + N0 // 2, z + N1 // 2
+        x0, x1 = max(int(np.floor(cx)) - r, 0), min(int(np.ceil(cx)) + r + 1, N0)
+        z0, z1 = max(int(np.floor(cz)) - r, 0), min(int(np.ceil(cz)) + r + 1, N1)
+        if x1 <= x0 or z1 <= z0:
+            continue
+        gx = np.exp(-(np.arange(x0, x1) - cx) ** 2 / (2 * s * s))
+        gz = np.exp(-(np.arange(z0, z1) - cz) ** 2 / (2 * s * s))
+        out[keep, x0:x1, z0:z1] += wy[keep, None, None] * (gx[:, None] * gz[None, :])[None]
+    return out.astype(np.float32)

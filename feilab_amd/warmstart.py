@@ -4,10 +4,11 @@ A plain ICON solve (MainCode.cu:113-135, Iteration2.cu:33-146) starts from
 m = 0 and runs pre + icon + post steepest-descent iterations.  The learned
 start runs the first `start` iterations, maps that iterate through the U-Net
 (icon_unet_warm_start: m + s U(m / s), s the slice RMS) and continues with
-unthresholded descent iterations from the prediction until the data residual
-R(m) reaches a target, e.g. the residual plain ICON ends with.  Everything
-runs on the device through the C ABI; plain ICON (m_init = NULL) stays the
-fallback.
+unthresholded descent iterations from the prediction until every slice's data
+residual R(m) is at or below a target (the residual plain ICON ends with) and
+the reprojection FRC at the min-|theta| tilt is within a tolerance of plain
+ICON's.  Everything runs on the device through the C ABI; plain ICON
+(m_init = NULL) stays the fallback for any slice that misses the target.
 
 Training pairs are (m_start, m_final) from the device solve itself: the
 continuation of an iterate under the rest of the schedule is the uninterrupted
@@ -120,18 +121,31 @@ class LearnedSolver:
         self.d_start = DeviceArray(shape, np.float32)
         self.d_init = DeviceArray(shape, np.float32)
 
-    def run(self, S, d_sino, d_m, cont, d_resid=None, stream=None):
+    def run(self, S, d_sino, d_m, cont, d_resid=None, d_reproj=None, stream=None, use_net=True):
         """start plain iterations -> U-Net -> `cont` descent iterations into d_m;
-        d_resid [S][cont + 1] receives R(m) along the continuation."""
+        d_resid [S][cont + 1] receives R(m) along the continuation, d_reproj
+        [S][N0] the reprojections of the result.  use_net=False continues from
+        the start iterate itself (the same method without the network)."""
         p = self.plan
         p.reconstruct_dev(S, d_sino, self.d_start, SolveOptions(self.start, 0, 0, self.data_type, self.threshold, 1, 0),
                           stream=stream)
-        self.net.warm_start_dev(S, self.d_start, self.d_init, stream)
+        if use_net:
+            self.net.warm_start_dev(S, self.d_start, self.d_init, stream)
         p.reconstruct_dev(S, d_sino, d_m, SolveOptions(cont, 0, 0, self.data_type, self.threshold, 1, 1),
-                          d_m_init=self.d_init, d_resid=d_resid, stream=stream)
+                          d_m_init=self.d_init if use_net else self.d_start, d_resid=d_resid, d_reproj=d_reproj,
+                          stream=stream)
 
     def close(self):
         self.net.close()
+
+
+def reprojection_frc(reproj, sino, iomit, band):
+    """ICON's fullRec FRC (calFRC_nonsqure over the slice stack) of the
+    reprojections [S][N0] against the measured projection rows at the
+    min-|theta| tilt, averaged over rings band[0] .. band[1]-1."""
+    from .frc import frc
+    c = frc(np.asarray(reproj, np.float64), np.asarray(sino)[:, iomit, :])
+    return c, float(c[band[0]:band[1]].mean())
 
 
 def iterations_to_target(resid, target):
