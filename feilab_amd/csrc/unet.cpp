@@ -85,6 +85,7 @@ struct icon_unet {
   T16 x8, a1, cat1, p1, a2, cat2, p2, a3, cat3, p3, ab, bb, d3a, d3b, d2a, d2b, d1a, d1b;
   float* out = nullptr;      // [maxB][H][W] network output of the warm start
   float* scale = nullptr;    // [maxB] per-image input scale of the warm start
+  double* rms_part = nullptr;  // [maxB][image_rms_parts()] partial sums of the scale
   // gradients (fp32)
   T32 g_d1b, g_d1a, g_cat1, g_d2b, g_d2a, g_cat2, g_d3b, g_d3a, g_cat3, g_bb, g_ab, g_p3, g_a3, g_p2, g_a2, g_p1,
       g_a1, g_out;
@@ -187,7 +188,8 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   const hipError_t ce = in_scale ? launch_to_bf16_scaled(d_in, in_scale, (long)H * W, B, n->x8.p, st)
                                  : launch_f32_to_bf16(d_in, n->x8.p, P, 8, 0, 1, st);
   if (ce != hipSuccess) return ufail(2, "input conversion: %s", hipGetErrorString(ce));
-  R(conv(n, cv[E1A], n->x8.p, 8, 0, n->a1.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
+  UCHK(launch_conv_first(n->x8.p, 8, cv[E1A].w16, cv[E1A].cin, n->params + cv[E1A].off_b, n->a1.p, 32, B, H, W, 32,
+                         st));
   R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, 96, 64, B, H, W, true, nullptr, 0, 0, false, false, st));
   R(pool(n->cat1, 64, 32, n->p1, B, H, W, st));
   R(conv(n, cv[E2A], n->p1.p, 32, 0, n->a2.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
@@ -326,7 +328,8 @@ int icon_unet_create(icon_unet** out, int device, int max_batch, int H, int W, i
   if ((rc = alloc_bytes(n, (void**)&n->d_loss, sizeof(double)))) return bail(rc);
   const long P = (long)max_batch * H * W, P2 = P / 4, P4 = P / 16, P8 = P / 64;
   if ((rc = alloc_bytes(n, (void**)&n->out, sizeof(float) * P)) ||
-      (rc = alloc_bytes(n, (void**)&n->scale, sizeof(float) * max_batch)))
+      (rc = alloc_bytes(n, (void**)&n->scale, sizeof(float) * max_batch)) ||
+      (rc = alloc_bytes(n, (void**)&n->rms_part, sizeof(double) * max_batch * image_rms_parts())))
     return bail(rc);
   if ((rc = t16(n, n->x8, P, 8)) || (rc = t16(n, n->a1, P, 32)) || (rc = t16(n, n->cat1, P, 96)) ||
       (rc = t16(n, n->p1, P2, 32)) || (rc = t16(n, n->a2, P2, 64)) || (rc = t16(n, n->cat2, P2, 192)) ||
@@ -413,7 +416,7 @@ int icon_unet_warm_start(icon_unet* n, int S, const float* d_m, float* d_out, vo
   for (int s0 = 0; s0 < S; s0 += n->maxB) {
     const int B = std::min(n->maxB, S - s0);
     const float* m = d_m + (long)s0 * P;
-    UCHK(launch_image_rms(m, P, B, n->scale, st));
+    UCHK(launch_image_rms(m, P, B, n->rms_part, n->scale, st));
     int rc = forward(n, B, m, n->out, st, n->scale);
     if (rc) return rc;
     UCHK(launch_residual_add(m, n->out, n->scale, P, B, d_out + (long)s0 * P, st));

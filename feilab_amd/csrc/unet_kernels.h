@@ -74,7 +74,12 @@ hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float*
 hipError_t launch_f32_to_bf16(const float* in, uint16_t* out, long n, int ldo, int co0, int C, hipStream_t st);
 hipError_t launch_mse(const float* pred, const float* target, long P, float* dpred, double* loss, float scale,
                       hipStream_t st);
-hipError_t launch_image_rms(const float* in, long P, int B, float* scale, hipStream_t st);
+// scale[b] = RMS of image b (1 if zero); part holds B * image_rms_parts() doubles
+hipError_t launch_image_rms(const float* in, long P, int B, double* part, float* scale, hipStream_t st);
+int image_rms_parts();
+// First 3x3 convolution of a one-channel input (channel 0 of ldx channels), ReLU
+hipError_t launch_conv_first(const uint16_t* x, int ldx, const uint16_t* w, int Cin_w, const float* bias, uint16_t* y,
+                             int ldy, int N, int H, int W, int Cout, hipStream_t st);
 hipError_t launch_to_bf16_scaled(const float* in, const float* scale, long P, int B, uint16_t* out, hipStream_t st);
 hipError_t launch_residual_add(const float* m, const float* y, const float* scale, long P, int B, float* out,
                                hipStream_t st);

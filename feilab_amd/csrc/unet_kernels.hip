@@ -8,7 +8,7 @@
 // v_mfma_f32_16x16x32_bf16: M = pixels, N = output channels, K = taps x input
 // channels in groups of 32.  Lane l feeds A[pixel l%16][8 channels
 // 8*(l/16)...] and B[8 channels 8*(l/16)...][output channel l%16] as 16-byte
-// vectors straight from NHWC memory and receives D[pixel 4*(l/16)+i][channel
+// vectors from LDS-staged tiles and receives D[pixel 4*(l/16)+i][channel
 // l%16].  The same kernel computes the input gradient with the spatially
 // flipped, channel-transposed weights; the weight gradient is a split-K GEMM
 // over pixels whose operands are transposed through LDS.
@@ -70,74 +70,95 @@ __device__ __forceinline__ float load1(const float* p) { return *p; }
 __device__ __forceinline__ void store1(float* p, float v) { *p = v; }
 
 // ----------------------------------------------------------------- conv
-// Block: 256 threads = 4 waves; tile 64 pixels x 64 output channels; wave w
-// computes pixels 32*(w&1).. x channels 32*(w>>1).. as 2 x 2 MFMA tiles.
-template <typename TX, typename TY>
+// Block: 256 threads = 4 waves on a 16 x 16 pixel tile of one image and COB
+// output channels.  Per 32-channel chunk of the input the block stages the
+// tile with its one-pixel halo (18 x 18 pixels) and the chunk's weights for
+// all taps in LDS (rows padded to 80 bytes, so the 16 rows a 16-lane group
+// reads with ds_read_b128 fall in distinct banks); wave w then multiplies
+// tile rows 4w..4w+3 (4 MFMA row tiles of 16 pixels) by the COB/16 channel
+// tiles for every tap, reading both operands from LDS.
+constexpr int kTile = 16;
+constexpr int kHalo = kTile + 2;
+constexpr int kLdsRow = 40;   // bf16 per staged pixel / weight row (32 + 8 pad)
+
+template <typename TX, typename TY, int COB>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs<TX, TY> a) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const long P = (long)a.N * a.H * a.W;
-  const long p_base = (long)blockIdx.x * 64 + 32 * (wave & 1);
-  const int co_base = blockIdx.y * 64 + 32 * (wave >> 1);
+  constexpr int CT = COB / 16;
+  __shared__ __align__(16) uint16_t s_in[kHalo * kHalo * kLdsRow];
+  __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
-  // the two pixels this lane feeds into A (one per 16-row sub-tile)
-  int pn[2], py[2], px[2];
-  bool pv[2];
+  const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
+  int bid = blockIdx.x;
+  const int tx = bid % tiles_x;
+  bid /= tiles_x;
+  const int ty = bid % tiles_y;
+  const int n = bid / tiles_y;
+  const int x0 = tx * kTile, y0 = ty * kTile;
+  const int co0 = blockIdx.y * COB;
+  const int taps = a.taps;
+  f4 acc[4][CT];
 #pragma unroll
-  for (int i = 0; i < 2; i++) {
-    const long p = p_base + 16 * i + r16;
-    pv[i] = p < P;
-    const long pp = pv[i] ? p : 0;
-    px[i] = (int)(pp % a.W);
-    py[i] = (int)((pp / a.W) % a.H);
-    pn[i] = (int)(pp / ((long)a.W * a.H));
-  }
-  f4 acc[2][2];
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const int half = a.taps == 9 ? 1 : 0;
-  for (int tap = 0; tap < a.taps; tap++) {
-    const int dy = tap / 3 * half - half, dx = tap % 3 * half - half;
-    const TX* src[2];
-    bool ok[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      const int yy = py[i] + dy, xx = px[i] + dx;
-      ok[i] = pv[i] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-      src[i] = a.x + (((long)pn[i] * a.H + (ok[i] ? yy : 0)) * a.W + (ok[i] ? xx : 0)) * a.ldx + a.cx0;
+    for (int j = 0; j < CT; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
+    // input tile + halo: kHalo^2 pixels x 4 chunks of 8 channels
+    for (int q = tid; q < kHalo * kHalo * 4; q += 256) {
+      const int c = q & 3, pix = q >> 2;
+      const int yy = y0 + pix / kHalo - 1, xx = x0 + pix % kHalo - 1;
+      const int ci = ci0 + 8 * c;
+      const bool ok = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ci < a.Cin;
+      const TX* src = a.x + (((long)n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.ldx + a.cx0 + (ok ? ci : 0);
+      V8 v;
+      v.v = load8<TX>(src, ok);
+      *reinterpret_cast<uint4*>(&s_in[pix * kLdsRow + 8 * c]) = v.u;
     }
-    const uint16_t* wt = a.w + (long)tap * a.Cout * a.Cin;
-    for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
-      const int ci = ci0 + 8 * g;
-      const bool cok = ci < a.Cin;
-      bf8 A[2], B[2];
+    // weights of every tap: taps x COB rows x 4 chunks
+    for (int q = tid; q < taps * COB * 4; q += 256) {
+      const int c = q & 3, r = q >> 2;
+      const int co = co0 + r % COB, t = r / COB, ci = ci0 + 8 * c;
+      const bool ok = co < a.Cout && ci < a.Cin;
+      const uint4 v = ok ? *reinterpret_cast<const uint4*>(a.w + ((long)t * a.Cout + co) * a.Cin + ci)
+                         : make_uint4(0, 0, 0, 0);
+      *reinterpret_cast<uint4*>(&s_w[r * kLdsRow + 8 * c]) = v;
+    }
+    __syncthreads();
+    for (int t = 0; t < taps; t++) {
+      const int hy = taps == 9 ? t / 3 : 1, hx = taps == 9 ? t % 3 : 1;   // halo offsets
+      bf8 B[CT];
 #pragma unroll
-      for (int i = 0; i < 2; i++) A[i] = load8<TX>(src[i] + ci, ok[i] && cok);
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int co = co_base + 16 * j + r16;
-        B[j] = load8<uint16_t>(wt + (long)co * a.Cin + ci, cok && co < a.Cout);
+      for (int j = 0; j < CT; j++) {
+        V8 v;
+        v.u = *reinterpret_cast<const uint4*>(&s_w[(t * COB + 16 * j + r16) * kLdsRow + 8 * g]);
+        B[j] = v.v;
       }
 #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int i = 0; i < 4; i++) {
+        V8 v;
+        v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + i + hy) * kHalo + r16 + hx) * kLdsRow + 8 * g]);
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v.v, B[j], acc[i][j], 0, 0, 0);
+      }
     }
+    __syncthreads();
   }
-  // epilogue: bias, ReLU, gradient mask, store
+  // epilogue: D[pixel 4g + r of row tile i][channel r16 of channel tile j]
 #pragma unroll
-  for (int j = 0; j < 2; j++) {
-    const int co = co_base + 16 * j + r16;
+  for (int j = 0; j < CT; j++) {
+    const int co = co0 + 16 * j + r16;
     if (co >= a.Cout) continue;
     const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < 4; i++) {
+      const int y = y0 + 4 * wave + i;
+      if (y >= a.H) continue;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const long p = p_base + 16 * i + 4 * g + r;
-        if (p >= P) continue;
+        const int x = x0 + 4 * g + r;
+        if (x >= a.W) continue;
+        const long p = ((long)n * a.H + y) * a.W + x;
         float v = acc[i][j][r] + bias;
         if (a.relu) v = fmaxf(v, 0.f);
         if (a.mask && !(bf2f(a.mask[p * a.ldm + a.cm0 + co]) > 0.f)) v = 0.f;
@@ -255,18 +276,26 @@ __global__ __launch_bounds__(256) void k_bias_grad(const float* dy, int ld, int 
   if (part == 0 && c < C) atomicAdd(&db[c], red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
-// 2x2 average pool (bf16 NHWC, channel sub-ranges)
+// 2x2 average pool (bf16 NHWC, channel sub-ranges); 8 channels per thread
+// (C, strides and channel offsets are multiples of 8).
 __global__ void k_pool_fwd(PoolArgs a) {
-  const long n = (long)a.N * (a.H / 2) * (a.W / 2) * a.C;
+  const int C8 = a.C / 8;
+  const long n = (long)a.N * (a.H / 2) * (a.W / 2) * C8;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % a.C);
-    const long q = i / a.C;
+    const int c = 8 * (int)(i % C8);
+    const long q = i / C8;
     const int x = (int)(q % (a.W / 2)), y = (int)((q / (a.W / 2)) % (a.H / 2));
     const long nn = q / ((long)(a.W / 2) * (a.H / 2));
     const long b = ((nn * a.H + 2 * y) * a.W + 2 * x) * a.ldi + a.ci0 + c;
-    const float s = bf2f(a.in[b]) + bf2f(a.in[b + a.ldi]) + bf2f(a.in[b + (long)a.W * a.ldi]) +
-                    bf2f(a.in[b + (long)a.W * a.ldi + a.ldi]);
-    a.out[q * a.ldo + a.co0 + c] = f2bf(0.25f * s);
+    V8 v0, v1, v2, v3, o;
+    v0.u = *reinterpret_cast<const uint4*>(a.in + b);
+    v1.u = *reinterpret_cast<const uint4*>(a.in + b + a.ldi);
+    v2.u = *reinterpret_cast<const uint4*>(a.in + b + (long)a.W * a.ldi);
+    v3.u = *reinterpret_cast<const uint4*>(a.in + b + (long)a.W * a.ldi + a.ldi);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      o.h[k] = f2bf(0.25f * (bf2f(v0.h[k]) + bf2f(v1.h[k]) + bf2f(v2.h[k]) + bf2f(v3.h[k])));
+    *reinterpret_cast<uint4*>(a.out + q * a.ldo + a.co0 + c) = o.u;
   }
 }
 
@@ -284,15 +313,17 @@ __global__ void k_pool_bwd(PoolGradArgs a) {
   }
 }
 
-// nearest 2x upsample (bf16): out[2y+u][2x+v] = in[y][x]
+// nearest 2x upsample (bf16): out[2y+u][2x+v] = in[y][x]; 8 channels per thread
 __global__ void k_up_fwd(PoolArgs a) {
-  const long n = (long)a.N * (2 * a.H) * (2 * a.W) * a.C;   // H, W of the input
+  const int C8 = a.C / 8;
+  const long n = (long)a.N * (2 * a.H) * (2 * a.W) * C8;   // H, W of the input
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % a.C);
-    const long q = i / a.C;
+    const int c = 8 * (int)(i % C8);
+    const long q = i / C8;
     const int x = (int)(q % (2 * a.W)), y = (int)((q / (2 * a.W)) % (2 * a.H));
     const long nn = q / ((long)(2 * a.W) * (2 * a.H));
-    a.out[q * a.ldo + a.co0 + c] = a.in[((nn * a.H + y / 2) * a.W + x / 2) * a.ldi + a.ci0 + c];
+    *reinterpret_cast<uint4*>(a.out + q * a.ldo + a.co0 + c) =
+        *reinterpret_cast<const uint4*>(a.in + ((nn * a.H + y / 2) * a.W + x / 2) * a.ldi + a.ci0 + c);
   }
 }
 
@@ -321,12 +352,52 @@ __global__ void k_relu_mask(float* g, int ldg, int cg0, const uint16_t* act, int
   }
 }
 
-// 1x1 head: out[p] = sum_c x[p][c] w[c] + b  (fp32 out)
+// 1x1 head: out[p] = sum_c x[p][c] w[c] + b  (fp32 out); C a multiple of 8
 __global__ void k_head_fwd(const uint16_t* x, int ldx, const float* w, const float* b, int C, long P, float* out) {
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
     float s = b[0];
-    for (int c = 0; c < C; c++) s = fmaf(bf2f(x[p * ldx + c]), w[c], s);
+    for (int c = 0; c < C; c += 8) {
+      V8 v;
+      v.u = *reinterpret_cast<const uint4*>(x + p * ldx + c);
+#pragma unroll
+      for (int k = 0; k < 8; k++) s = fmaf(bf2f(v.h[k]), w[c + k], s);
+    }
     out[p] = s;
+  }
+}
+
+// First convolution, whose input has one real channel (channel 0 of an
+// ldx-channel bf16 tensor): y[p][co] = relu(bias[co] + sum_t w[t][co][0] x[p + off(t)][0]),
+// 8 output channels per 16-byte store.  Memory-bound, so plain FP32 FMAs.
+__global__ __launch_bounds__(256) void k_conv_first(const uint16_t* x, int ldx, const uint16_t* w, int Cin_w,
+                                                   const float* bias, uint16_t* y, int ldy, int N, int H, int W,
+                                                   int Cout) {
+  __shared__ float s_w[9 * 64];
+  __shared__ float s_b[64];
+  for (int i = threadIdx.x; i < 9 * Cout; i += blockDim.x) s_w[i] = bf2f(w[(long)i * Cin_w]);
+  for (int i = threadIdx.x; i < Cout; i += blockDim.x) s_b[i] = bias[i];
+  __syncthreads();
+  const long P = (long)N * H * W;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+    const int px = (int)(p % W), py = (int)((p / W) % H);
+    const long pn = p / ((long)W * H);
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+      const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
+      v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? bf2f(x[((pn * H + yy) * W + xx) * ldx]) : 0.f;
+    }
+    for (int c = 0; c < Cout; c += 8) {
+      V8 o;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        float acc = 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) acc = fmaf(v[t], s_w[t * Cout + c + k], acc);
+        o.h[k] = f2bf(fmaxf(acc + s_b[c + k], 0.f));
+      }
+      *reinterpret_cast<uint4*>(y + p * ldy + c) = o.u;
+    }
   }
 }
 
@@ -412,22 +483,32 @@ __global__ __launch_bounds__(256) void k_mse(const float* pred, const float* tar
 }
 
 // Per-image scale of the warm start: s = RMS of the image (1 for a zero image),
-// accumulated in FP64.  One block per image.
-__global__ __launch_bounds__(256) void k_image_rms(const float* in, long P, float* scale) {
+// accumulated in FP64: kRmsParts blocks per image write partial sums, then one
+// thread per image adds them in a fixed order.
+constexpr int kRmsParts = 64;
+
+__global__ __launch_bounds__(256) void k_image_rms_part(const float* in, long P, double* part) {
   __shared__ double red[256];
-  const float* x = in + (long)blockIdx.x * P;
+  const float* x = in + (long)blockIdx.y * P;
   double s = 0.0;
-  for (long p = threadIdx.x; p < P; p += blockDim.x) s += (double)x[p] * x[p];
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x)
+    s += (double)x[p] * x[p];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double r = sqrt(red[0] / (double)P);
-    scale[blockIdx.x] = r > 0.0 ? (float)r : 1.f;
-  }
+  if (threadIdx.x == 0) part[(long)blockIdx.y * kRmsParts + blockIdx.x] = red[0];
+}
+
+__global__ void k_image_rms_final(const double* part, long P, int B, float* scale) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double s = 0.0;
+  for (int k = 0; k < kRmsParts; k++) s += part[(long)b * kRmsParts + k];
+  const double r = sqrt(s / (double)P);
+  scale[b] = r > 0.0 ? (float)r : 1.f;
 }
 
 // out[p][0] = bf16(in[p] / scale[image]) into an 8-channel NHWC buffer
@@ -452,9 +533,13 @@ static int blocks_for(long n, int per = 256) { return (int)std::min<long>((n + p
 
 template <typename TX, typename TY>
 hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
-  const long P = (long)a.N * a.H * a.W;
-  dim3 grid((unsigned)((P + 63) / 64), (unsigned)((a.Cout + 63) / 64));
-  hipLaunchKernelGGL((k_conv<TX, TY>), grid, dim3(256), 0, st, a);
+  if (a.taps != 9 && a.taps != 1) return hipErrorInvalidValue;
+  const long tiles = (long)a.N * ((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
+  if (a.Cout <= 32) {
+    hipLaunchKernelGGL((k_conv<TX, TY, 32>), dim3((unsigned)tiles, (unsigned)((a.Cout + 31) / 32)), dim3(256), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_conv<TX, TY, 64>), dim3((unsigned)tiles, (unsigned)((a.Cout + 63) / 64)), dim3(256), 0, st, a);
+  }
   return hipGetLastError();
 }
 template hipError_t launch_conv<uint16_t, uint16_t>(const ConvArgs<uint16_t, uint16_t>&, hipStream_t);
@@ -472,7 +557,7 @@ hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, floa
 }
 
 hipError_t launch_pool_fwd(const PoolArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C / 4)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C / 32)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 hipError_t launch_pool_bwd(const PoolGradArgs& a, hipStream_t st) {
@@ -480,7 +565,7 @@ hipError_t launch_pool_bwd(const PoolGradArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_up_fwd(const PoolArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_up_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C * 4)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_up_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C / 2)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 hipError_t launch_up_bwd(const PoolGradArgs& a, hipStream_t st) {
@@ -522,8 +607,17 @@ hipError_t launch_mse(const float* pred, const float* target, long P, float* dpr
   hipLaunchKernelGGL(k_mse, dim3(blocks_for(P, 1024)), dim3(256), 0, st, pred, target, P, dpred, loss, scale);
   return hipGetLastError();
 }
-hipError_t launch_image_rms(const float* in, long P, int B, float* scale, hipStream_t st) {
-  hipLaunchKernelGGL(k_image_rms, dim3(B), dim3(256), 0, st, in, P, scale);
+hipError_t launch_image_rms(const float* in, long P, int B, double* part, float* scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_image_rms_part, dim3(kRmsParts, B), dim3(256), 0, st, in, P, part);
+  hipLaunchKernelGGL(k_image_rms_final, dim3((B + 63) / 64), dim3(64), 0, st, part, P, B, scale);
+  return hipGetLastError();
+}
+int image_rms_parts() { return kRmsParts; }
+hipError_t launch_conv_first(const uint16_t* x, int ldx, const uint16_t* w, int Cin_w, const float* bias, uint16_t* y,
+                             int ldy, int N, int H, int W, int Cout, hipStream_t st) {
+  if (Cout > 64 || Cout % 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_conv_first, dim3(blocks_for((long)N * H * W)), dim3(256), 0, st, x, ldx, w, Cin_w, bias, y,
+                     ldy, N, H, W, Cout);
   return hipGetLastError();
 }
 hipError_t launch_to_bf16_scaled(const float* in, const float* scale, long P, int B, uint16_t* out, hipStream_t st) {
