@@ -158,19 +158,32 @@ __device__ __forceinline__ void pass_store(const float2 (&v)[16], float2* lds, i
   }
 }
 
+// Barrier between the LDS exchanges of a line: the whole block, or only the
+// wavefront when one wave owns the line (WAVE = 1).
+template <int WAVE>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Runs the passes from sub-transform size NS on; v holds that pass's inputs.
-// Every thread of the block must call this (it contains __syncthreads).
-template <int N, int ES, int DIR, int NS>
+// Every thread of the block (WAVE = 0) or of the wave (WAVE = 1) must call it.
+template <int N, int ES, int DIR, int NS, int WAVE = 0>
 __device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, const float2* __restrict__ tw) {
   constexpr int RP = pass_radix<N, NS>();
   pass_compute<N, RP, NS, DIR>(v, t, tw);
   if constexpr (NS * RP < N) {
     pass_store<N, RP, NS, ES>(v, lds, t);
-    __syncthreads();
+    line_sync<WAVE>();
     constexpr int RN = pass_radix<N, NS * RP>();
     pass_load<N, RN, ES>(v, lds, t);
-    __syncthreads();
-    fft_from<N, ES, DIR, NS * RP>(v, lds, t, tw);
+    line_sync<WAVE>();
+    fft_from<N, ES, DIR, NS * RP, WAVE>(v, lds, t, tw);
   }
 }
 
@@ -205,6 +218,20 @@ __device__ __forceinline__ void fft_lds(float2* lds, int t, const float2* __rest
   fft_regs<N, ES, DIR>(v, lds, t, tw);
   // the last pass_load is followed by a barrier inside fft_from, so the
   // line can be overwritten now
+#pragma unroll
+  for (int i = 0; i < 16; i++) lds[out_index<N>(t, i) * ES] = v[i];
+}
+
+// The same with one wavefront per line (N / 16 == 64 threads, t = lane):
+// the exchanges need only wavefront barriers.  The caller synchronises the
+// block before (the line is complete in LDS) and after (it is read by others).
+template <int N, int ES, int DIR>
+__device__ __forceinline__ void fft_lds_wave(float2* lds, int t, const float2* __restrict__ tw) {
+  static_assert(N / 16 == 64, "one wavefront per line");
+  float2 v[16];
+  pass_load<N, 16, ES>(v, lds, t);
+  line_sync<1>();
+  fft_from<N, ES, DIR, 1, 1>(v, lds, t, tw);
 #pragma unroll
   for (int i = 0; i < 16; i++) lds[out_index<N>(t, i) * ES] = v[i];
 }

@@ -149,12 +149,19 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
 // x0 + tx and the group sums its 14 rows with a 16-lane shuffle reduction.
 // Spread: see k_col_adj (row-owner gather, no atomics).
 
+// Row stride (float2) of the LDS tile: C + 2 keeps rows 16-byte aligned for
+// vector reads while two blocks still share a CU's LDS; otherwise C + 1.
+template <int NC, int C>
+__host__ __device__ constexpr int col_stride() {
+  return NC * (C + 2) * 8 <= 81920 ? C + 2 : C + 1;
+}
+
 template <int NC, int C>
 __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   constexpr int NT = C * (NC / 16);
-  constexpr int LS = C + 1;
+  constexpr int LS = col_stride<NC, C>();
   constexpr int NG = NT / 16;
-  extern __shared__ float2 tile[];   // NC * LS
+  extern __shared__ __align__(16) float2 tile[];   // NC * LS
   // tile-major order: blocks in flight together work on the same column
   // tile of different slices and share its bin records in L2
   const int blk = blockIdx.x / a.S;
@@ -162,11 +169,21 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int h0 = a.N0 >> 1;
-  // load the non-zero rows (zero padding elsewhere)
-  for (int i = tid; i < NC * C; i += NT) {
+  // load the non-zero rows (zero padding elsewhere): every load in flight
+  // before the first LDS store
+  constexpr int LPT = NC * C / NT;   // 16
+  float2 ld[LPT];
+#pragma unroll
+  for (int k = 0; k < LPT; k++) {
+    const int i = tid + k * NT;
     const int x = i / C, c = i % C;
     const int X = (x + h0) & (NC - 1);
-    tile[x * LS + c] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
+    ld[k] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < LPT; k++) {
+    const int i = tid + k * NT;
+    tile[(i / C) * LS + i % C] = ld[k];
   }
   __syncthreads();
   fft_lds<NC, LS, -1>(tile + (tid % C), tid / C, a.tw);
@@ -189,12 +206,24 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
     float ax = 0.f, ay = 0.f;
     if (lane < kTapsK) {
       const float2* row = tile + ((en.z + lane) & (NC - 1)) * LS;
+      float2 gv[C];
+      if constexpr (LS % 2 == 0) {
+        // 16-byte aligned rows: two cells per LDS read
+#pragma unroll
+        for (int c2 = 0; c2 < C / 2; c2++) {
+          const float4 q = reinterpret_cast<const float4*>(row)[c2];
+          gv[2 * c2] = make_float2(q.x, q.y);
+          gv[2 * c2 + 1] = make_float2(q.z, q.w);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; c++) gv[c] = row[c];
+      }
 #pragma unroll
       for (int c = 0; c < C; c++) {
         const float phi = px * py[c];
-        const float2 gv = row[c];
-        ax = __fmaf_rn(gv.x, phi, ax);
-        ay = __fmaf_rn(gv.y, phi, ay);
+        ax = __fmaf_rn(gv[c].x, phi, ax);
+        ay = __fmaf_rn(gv[c].y, phi, ay);
       }
     }
 #pragma unroll
@@ -223,13 +252,16 @@ __device__ __forceinline__ void wave_sync() {
 template <int NC, int C>
 __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
   constexpr int NT = C * (NC / 16);
-  constexpr int LS = C + 1;
+  constexpr int LS = col_stride<NC, C>();
   constexpr int NW = NT / 64;                    // waves per block
   constexpr int RPT = NC >= NT ? NC / NT : 1;    // grid rows per thread
-  constexpr int REC = kTapsK + C + 3;            // staged record: x-taps, y-taps, f, first x-tap
+  // staged record, 16-byte aligned fields: x-taps [0,14) + 2 pad, y-taps
+  // [16, 16+C), then (f.x, f.y, first x-tap, pad)
+  constexpr int REC = 16 + C + 4;
   constexpr int REGION = NC * LS * 2 / NW;       // floats of LDS staging per wave
-  static_assert(64 * REC <= REGION, "staging chunk does not fit");
-  extern __shared__ float2 tile[];   // NC * LS
+  constexpr int CH = 64 * REC <= REGION ? 64 : 32;   // records staged per chunk
+  static_assert(CH * REC <= REGION && REGION % 4 == 0, "staging chunk does not fit");
+  extern __shared__ __align__(16) float2 tile[];   // NC * LS
   // tile-major order: blocks in flight together work on the same column
   // tile of different slices and share its bin records in L2
   const int blk = blockIdx.x / a.S;
@@ -280,25 +312,24 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
       const int lo = run ? lo2 : lo1, hi = run ? hi2 : hi1;
       // the wrap run of the first wave is run 0; otherwise the single run is run 1
       const int gr = (xa >= kTapsK - 1) ? 1 : run;
-      for (int base = lo; base < hi; base += 64) {
-        const int cnt = min(64, hi - base);
+      for (int base = lo; base < hi; base += CH) {
+        const int cnt = min(CH, hi - base);
         wave_sync();
         if (lane < cnt) {
           const int e = base + lane;
-          float* rec = stage + lane * REC;
+          float4* rec = reinterpret_cast<float4*>(stage + lane * REC);
+          const float2* pxv = reinterpret_cast<const float2*>(a.ent_px + (size_t)e * kTapsK);
+          float2 p[kTapsK / 2];
 #pragma unroll
-          for (int q = 0; q < kTapsK; q++) rec[q] = a.ent_px[(size_t)e * kTapsK + q];
+          for (int q = 0; q < kTapsK / 2; q++) p[q] = pxv[q];
+#pragma unroll
+          for (int q = 0; q < 3; q++) rec[q] = make_float4(p[2 * q].x, p[2 * q].y, p[2 * q + 1].x, p[2 * q + 1].y);
+          rec[3] = make_float4(p[6].x, p[6].y, 0.f, 0.f);
           const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
 #pragma unroll
-          for (int c4 = 0; c4 < C / 4; c4++) {
-            const float4 v = pyv[c4];
-            rec[kTapsK + 4 * c4] = v.x; rec[kTapsK + 4 * c4 + 1] = v.y;
-            rec[kTapsK + 4 * c4 + 2] = v.z; rec[kTapsK + 4 * c4 + 3] = v.w;
-          }
+          for (int c4 = 0; c4 < C / 4; c4++) rec[4 + c4] = pyv[c4];
           const float2 f = fw[e];
-          rec[kTapsK + C] = f.x;
-          rec[kTapsK + C + 1] = f.y;
-          rec[kTapsK + C + 2] = __int_as_float(a.ent[e].z);
+          rec[4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
         }
         wave_sync();
         // the group's part of this chunk
@@ -306,15 +337,21 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
         const int k1 = min(ghi[gr], base + cnt) - base;
         auto add = [&](int k) {
           const float* rec = stage + k * REC;
-          const int tx = (x - __float_as_int(rec[kTapsK + C + 2])) & (NC - 1);
+          const float4 tail = reinterpret_cast<const float4*>(rec)[4 + C / 4];
+          const int tx = (x - __float_as_int(tail.z)) & (NC - 1);
           if (tx < kTapsK) {
             const float px = rec[tx];
-            const float fx = rec[kTapsK + C], fy = rec[kTapsK + C + 1];
+            float py[C];
+#pragma unroll
+            for (int c4 = 0; c4 < C / 4; c4++) {
+              const float4 v = reinterpret_cast<const float4*>(rec)[4 + c4];
+              py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
+            }
 #pragma unroll
             for (int c = 0; c < C; c++) {
-              const float phi = px * rec[kTapsK + c];
-              acc[r][c].x = __fmaf_rn(fx, phi, acc[r][c].x);
-              acc[r][c].y = __fmaf_rn(fy, phi, acc[r][c].y);
+              const float phi = px * py[c];
+              acc[r][c].x = __fmaf_rn(tail.x, phi, acc[r][c].x);
+              acc[r][c].y = __fmaf_rn(tail.y, phi, acc[r][c].y);
             }
           }
         };
@@ -705,10 +742,10 @@ static hipError_t col_attr() {
   static bool done = false;
   if (!done) {
     hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       NC * (C + 1) * (int)sizeof(float2));
+                                       NC * col_stride<NC, C>() * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_col_adj<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            NC * (C + 1) * (int)sizeof(float2));
+                            NC * col_stride<NC, C>() * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_fft_cols<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             NC * (C + 1) * (int)sizeof(float2));
@@ -735,10 +772,10 @@ hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t
     e = col_attr<N, CC>();                                                                               \
     if (e != hipSuccess) return e;                                                                       \
     if (adjoint)                                                                                         \
-      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (CC + 1) * sizeof(float2), \
+      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (col_stride<N, CC>()) * sizeof(float2), \
                          st, a);                                                                         \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (CC + 1) * sizeof(float2), \
+      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (col_stride<N, CC>()) * sizeof(float2), \
                          st, a);                                                                         \
     return hipGetLastError();                                                                            \
   }
