@@ -191,7 +191,8 @@ int op_adjoint(icon_plan* p, int S, const float2* f, float2* img, hipStream_t st
 int op_sino(icon_plan* p, int S, const float* sino, float2* b, hipStream_t st) {
   const Geometry& g = p->g;
   SinoArgs sa{};
-  sa.rows = S * g.nz; sa.N0 = g.N0; sa.sino = sino; sa.ck1 = p->d_ck1; sa.phi1 = p->d_phi1;
+  sa.rows = S * g.nz; sa.N0 = g.N0; sa.nz = g.nz; sa.skip = g.cv_skip; sa.sino = sino; sa.ck1 = p->d_ck1;
+  sa.phi1 = p->d_phi1;
   sa.i10 = p->d_i10; sa.tw = p->d_tw0; sa.b = b;
   return check_launch(launch_sino(g.n0, sa, st), "sino");
 }
@@ -366,8 +367,26 @@ int icon_geometry_tables(int nx, int thickness, int nz, const float* tilt_deg, i
   return ICON_OK;
 }
 
+namespace {
+int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg, int max_slices,
+                bool cv);
+}
+
 int icon_plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg,
                      int max_slices) {
+  return plan_create(out, device, nx, thickness, nz, tilt_deg, max_slices, false);
+}
+
+int icon_plan_create_cv(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg,
+                        int max_slices) {
+  return plan_create(out, device, nx, thickness, nz, tilt_deg, max_slices, true);
+}
+
+}  // extern "C"
+
+namespace {
+int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg, int max_slices,
+                bool cv) {
   if (!out || !tilt_deg) return fail(ICON_ERR_ARG, "icon_plan_create: null pointer");
   *out = nullptr;
   if (max_slices < 1) return fail(ICON_ERR_ARG, "icon_plan_create: max_slices must be >= 1");
@@ -379,7 +398,7 @@ int icon_plan_create(icon_plan** out, int device, int nx, int thickness, int nz,
   auto* p = new icon_plan();
   p->device = device;
   std::string err;
-  if (!build_geometry(p->g, nx, thickness > 0 ? thickness : nx, nz, tilt_deg, &err)) {
+  if (!build_geometry(p->g, nx, thickness > 0 ? thickness : nx, nz, tilt_deg, &err, cv)) {
     delete p;
     return fail(ICON_ERR_ARG, "%s", err.c_str());
   }
@@ -427,6 +446,9 @@ int icon_plan_create(icon_plan** out, int device, int nx, int thickness, int nz,
   *out = p;
   return ICON_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int icon_plan_destroy(icon_plan* p) {
   if (!p) return ICON_OK;

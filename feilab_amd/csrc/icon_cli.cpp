@@ -39,6 +39,63 @@ void help() {
 
 }  // namespace
 
+// The cross-validation outputs (ICONGPU.cu:468-592): the measured projection
+// at the least-|angle| tilt for the reconstructed slices (GroundTruth.mrc),
+// the reprojections of the full and the cross-validation solves, and their
+// smoothed FRCs against the measurement (fullRec.frc, crossV.frc: "%f %f"
+// lines of spatial frequency i/nx and FRC, i < nx/2).
+static bool cross_validation_outputs(const std::string& input, const std::string& tilt, const std::string& cvdir,
+                                     const MrcHeader& head, int y0, int total, const std::vector<float>& full,
+                                     const std::vector<float>& cv) {
+  const int nx = head.nx;
+  std::vector<float> tl(1000);
+  int nt = 0;
+  if (icon_read_tilts(tilt.c_str(), tl.data(), (int)tl.size(), &nt) != ICON_OK || nt < 1) {
+    std::printf("cannot read %s\n", tilt.c_str());
+    return false;
+  }
+  float om = -tl[0];
+  int omit = 0;
+  for (int i = 0; i < nt; i++)
+    if (std::fabs(om) > std::fabs(-tl[i])) { om = -tl[i]; omit = i; }
+  std::vector<float> sec((size_t)nx * head.ny), gt((size_t)total * nx);
+  std::string err;
+  FILE* fp = std::fopen(input.c_str(), "rb");
+  const bool ok = fp && mrc::read_z_section(fp, head, omit, sec.data(), &err);
+  if (fp) std::fclose(fp);
+  if (!ok) {
+    std::printf("cannot read the projection at tilt %d: %s\n", omit, err.c_str());
+    return false;
+  }
+  for (int r = 0; r < total; r++)
+    std::memcpy(&gt[(size_t)r * nx], &sec[(size_t)(y0 + r) * nx], sizeof(float) * nx);
+  const struct { const char* name; const float* data; } files[] = {
+      {"GroundTruth.mrc", gt.data()}, {"fullRec_reProjection.mrc", full.data()}, {"crossV_reprojection.mrc", cv.data()}};
+  for (const auto& f : files) {
+    const std::string path = cvdir + "/" + f.name;
+    if (!mrc::write_real(path.c_str(), head, f.data, nx, total, &err)) {
+      std::printf("cannot write %s: %s\n", path.c_str(), err.c_str());
+      return false;
+    }
+  }
+  const double resstep = 1.0 / nx;
+  const struct { const char* name; const float* rec; } curves[] = {{"fullRec.frc", full.data()}, {"crossV.frc", cv.data()}};
+  std::vector<float> frc(nx / 2 + 1);
+  for (const auto& c : curves) {
+    if (icon_frc(gt.data(), c.rec, nx, total, frc.data()) != ICON_OK) return false;
+    icon_frc_smooth(frc.data(), nx / 2, 10);
+    const std::string path = cvdir + "/" + c.name;
+    FILE* out = std::fopen(path.c_str(), "w");
+    if (!out) {
+      std::printf("cannot write %s\n", path.c_str());
+      return false;
+    }
+    for (int i = 0; i < nx / 2; i++) std::fprintf(out, "%f %f\n", resstep * i, frc[i]);
+    std::fclose(out);
+  }
+  return true;
+}
+
 int main(int argc, char** argv) {
   std::string input, tilt, out;
   int y0 = 0, y1 = -1, pre = 0, mid = 0, post = 0, data_type = 1, thickness = 0, skip_cv = 0;
@@ -128,15 +185,18 @@ int main(int argc, char** argv) {
     }
     std::fclose(fp);
   }
-  std::vector<float> reproj((size_t)total * head.nx, 0.f);
+  std::vector<float> reproj((size_t)total * head.nx, 0.f), reproj_cv((size_t)total * head.nx, 0.f);
   std::vector<int> status(used, 0);
   std::vector<std::thread> th;
   for (int d = 0; d < used; d++) {
     std::printf("thread %d using GPU %d , deals with slices : start %d end %d\n", d, devices[d], first[d], last[d]);
     th.emplace_back([&, d]() {
-      status[d] = icon_lib_2d2(devices[d], first[d], last[d] + 1, iternum, input.c_str(),
-                               reproj.data() + (size_t)(first[d] - y0) * head.nx, tilt.c_str(), recdir.c_str(),
-                               before, betain, 0);
+      const size_t off = (size_t)(first[d] - y0) * head.nx;
+      status[d] = icon_lib_2d2(devices[d], first[d], last[d] + 1, iternum, input.c_str(), reproj.data() + off,
+                               tilt.c_str(), recdir.c_str(), before, betain, 0);
+      if (status[d] == ICON_OK && skip_cv == 0)   // ICONGPU.cu:52-54
+        status[d] = icon_lib_2d_crossv(devices[d], first[d], last[d] + 1, iternum, input.c_str(),
+                                       reproj_cv.data() + off, tilt.c_str(), recdir.c_str(), before, betain, 0);
     });
   }
   for (auto& t : th) t.join();
@@ -145,5 +205,6 @@ int main(int argc, char** argv) {
       std::printf("device %d failed: %s\n", devices[d], icon_lib_2d2_error());
       return 1;
     }
+  if (skip_cv == 0 && !cross_validation_outputs(input, tilt, cvdir, head, y0, total, reproj, reproj_cv)) return 1;
   return 0;
 }

@@ -66,9 +66,10 @@ void grid_size(int N, float* sigma, int* n) {
   *n = (int)std::ceil(s * (float)N);
 }
 
-bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err) {
-  if (N0 < 8 || N1 < 8 || nz < 2) {
-    if (err) *err = "icon_plan_create: need nx >= 8, thickness >= 8 and at least 2 tilts";
+bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv) {
+  if (N0 < 8 || N1 < 8 || nz < (cv ? 3 : 2)) {
+    if (err) *err = cv ? "icon_plan_create_cv: need nx >= 8, thickness >= 8 and at least 3 tilts"
+                       : "icon_plan_create: need nx >= 8, thickness >= 8 and at least 2 tilts";
     return false;
   }
   if (nz > 1000) {
@@ -93,7 +94,8 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
   window_shape(g.sig0, kWindowM, &g.b0, &g.bf0);
   window_shape(g.sig1, kWindowM, &g.b1, &g.bf1);
 
-  // angles are negated on read (MainCode.cu:42)
+  // angles are negated on read (MainCode.cu:42); the omitted tilt is the
+  // first of least |theta| (MainCode.cu:46-49)
   g.theta.resize(nz);
   for (int i = 0; i < nz; i++) g.theta[i] = -tilt_deg[i];
   float om = g.theta[0];
@@ -103,6 +105,19 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
   g.iomit = 0;
   for (int i = 0; i < nz; i++)
     if (g.theta[i] == om) { g.iomit = i; break; }
+  // cross-validation plan (MainCode_crossV.cu:45-60): the omitted tilt leaves
+  // the data; the remaining nz-1 tilts keep their order and their own weights,
+  // and the omitted angle is appended as a reprojection line of weight 0
+  // (its nodes then contribute exactly nothing to A^H W, R or alpha)
+  g.cv_skip = -1;
+  int nw = nz;   // tilts that carry data
+  if (cv) {
+    g.cv_skip = g.iomit;
+    for (int i = g.iomit; i < nz - 1; i++) g.theta[i] = g.theta[i + 1];
+    g.theta[nz - 1] = om;
+    g.iomit = nz - 1;
+    nw = nz - 1;
+  }
 
   // radial nodes: float accumulation of the radius, double trig
   const int M = g.M;
@@ -122,18 +137,19 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
     }
   }
 
-  // density weights: |ramp| x angular spacing, centre 180/(4 nz)
-  g.w.resize(M);
+  // density weights over the nw data tilts: |ramp| x angular spacing, centre
+  // 180/(4 nw); a cross-validation plan's reprojection line has weight 0
+  g.w.assign(M, 0.f);
   {
     const int center = N0 / 2;
     const float* th = g.theta.data();
-    for (int i = 0; i < nz; i++) {
-      const float span = (i == 0) ? (th[1] - th[0]) : (i == nz - 1) ? (th[nz - 2] - th[nz - 1])
+    for (int i = 0; i < nw; i++) {
+      const float span = (i == 0) ? (th[1] - th[0]) : (i == nw - 1) ? (th[nw - 2] - th[nw - 1])
                                                                      : (th[i - 1] - th[i + 1]);
-      const double mult = (i == 0 || i == nz - 1) ? 2.0 : 1.0;
+      const double mult = (i == 0 || i == nw - 1) ? 2.0 : 1.0;
       for (int k = 0; k < N0; k++) {
         float v;
-        if (k == center) v = (float)(180.0 / (4 * nz));
+        if (k == center) v = (float)(180.0 / (4 * nw));
         else v = (float)(mult * std::fabs((double)(span * (float)(k - center))));
         g.w[(size_t)i * N0 + k] = v;
       }

@@ -74,6 +74,7 @@ struct CombineArgs {
 
 struct SinoArgs {
   int rows, N0;
+  int nz, skip;          // tilts per slice; input row left out (cross validation), else -1
   const float* sino;     // [S][nz][N0]
   const float* ck1;
   const float* phi1;

@@ -486,13 +486,23 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
   const int row = blockIdx.x * RPB + team;
   const bool valid = row < a.rows;
   const int h = a.N0 >> 1;
+  // node row i of slice s reads input row i, or (cross validation) the rows
+  // around the left-out one, the last node row being the data-free
+  // reprojection line
+  int in_row = row;
+  bool has_data = valid;
+  if (a.skip >= 0) {
+    const int s = row / a.nz, i = row % a.nz;
+    has_data = valid && i < a.nz - 1;
+    in_row = s * a.nz + (i < a.skip ? i : i + 1);
+  }
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int x = in_index<NR>(t, r);
     const int X = (x + h) & (NR - 1);
     float gv = 0.f;
-    if (valid && X < a.N0) gv = a.sino[(size_t)row * a.N0 + X] / a.ck1[X];
+    if (has_data && X < a.N0) gv = a.sino[(size_t)in_row * a.N0 + X] / a.ck1[X];
     v[r] = make_float2(gv, 0.f);
   }
   float2* L = lds + team * NR;

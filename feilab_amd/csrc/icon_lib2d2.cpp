@@ -138,11 +138,14 @@ extern "C" int icon_partition_slices(int ndev, const int* cus, int y_start, int 
 
 
 
-extern "C" int icon_lib_2d2(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
-                            float* reproj, const char* angle_path, const char* result_dir, const int* before,
-                            const float* betain, int batch) {
+// Lib_2D2 (cv = false) and Lib_2D_crossV2 (cv = true): the same batched
+// solve, the cross-validation one on a plan without the least-|angle| tilt,
+// writing no slice files.
+static int lib_2d(bool cv, int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
+                  float* reproj, const char* angle_path, const char* result_dir, const int* before,
+                  const float* betain, int batch) {
   if (!mrc_path || !angle_path || !result_dir || !before || !betain) {
-    set_err("icon_lib_2d2: null argument");
+    set_err(cv ? "icon_lib_2d_crossv: null argument" : "icon_lib_2d2: null argument");
     return ICON_ERR_ARG;
   }
   FILE* fin = std::fopen(mrc_path, "rb");
@@ -188,7 +191,8 @@ extern "C" int icon_lib_2d2(int device, int slice_begin, int slice_end, int iter
   }
   const int B = batch > 0 ? (batch < total ? batch : total) : (total < 256 ? total : 256);
   icon_plan* plan = nullptr;
-  rc = icon_plan_create(&plan, device, nx, thickness, nz, tilts.data(), B);
+  rc = cv ? icon_plan_create_cv(&plan, device, nx, thickness, nz, tilts.data(), B)
+          : icon_plan_create(&plan, device, nx, thickness, nz, tilts.data(), B);
   if (rc) {
     std::fclose(fin);
     set_err(icon_last_error());
@@ -213,20 +217,132 @@ extern "C" int icon_lib_2d2(int device, int slice_begin, int slice_end, int iter
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int s = 0; s < cnt; s++) {
       const int Y = y0 + s;
-      std::printf("slice %d finished, time %lf\n", Y, dt / cnt);
-      char path[4096];
-      std::snprintf(path, sizeof path, "%s/mid%05d.mrc", result_dir, Y);
-      if (!mrc::write_icon_slice(path, head, m.data() + (size_t)s * nx * thickness, nx, thickness, &err)) {
-        rc = ICON_ERR_IO;
-        set_err(err);
-        break;
+      if (cv) {
+        std::printf("cross validation!\n");
+      } else {
+        std::printf("slice %d finished, time %lf\n", Y, dt / cnt);
+        char path[4096];
+        std::snprintf(path, sizeof path, "%s/mid%05d.mrc", result_dir, Y);
+        if (!mrc::write_icon_slice(path, head, m.data() + (size_t)s * nx * thickness, nx, thickness, &err)) {
+          rc = ICON_ERR_IO;
+          set_err(err);
+          break;
+        }
       }
       if (reproj) std::memcpy(reproj + (size_t)(Y - slice_begin) * nx, rp.data() + (size_t)s * nx, sizeof(float) * nx);
     }
     if (rc) break;
   }
-  std::printf("full reconstruction\n");
+  if (!cv) std::printf("full reconstruction\n");
   icon_plan_destroy(plan);
   std::fclose(fin);
   return rc;
+}
+
+extern "C" int icon_lib_2d2(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
+                            float* reproj, const char* angle_path, const char* result_dir, const int* before,
+                            const float* betain, int batch) {
+  return lib_2d(false, device, slice_begin, slice_end, iternum, mrc_path, reproj, angle_path, result_dir, before,
+                betain, batch);
+}
+
+extern "C" int icon_lib_2d_crossv(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
+                                  float* reproj, const char* angle_path, const char* result_dir, const int* before,
+                                  const float* betain, int batch) {
+  return lib_2d(true, device, slice_begin, slice_end, iternum, mrc_path, reproj, angle_path, result_dir, before,
+                betain, batch);
+}
+
+// ---------------------------------------------------------------- FRC
+// calFRC_nonsqure (calFRC.cu:93-174): ring r = floor(sqrt(di^2 + dj^2 (nx/ny)^2))
+// of the centred nx x ny spectrum, r < nx/2; frc[0] = 1.  The 2D transforms
+// are separable DFTs in double on the host (any nx, ny).
+static void dft2(const float* img, int nx, int ny, std::vector<double>& re, std::vector<double>& im) {
+  std::vector<double> cx(nx), sx(nx), cy(ny), sy(ny);
+  for (int k = 0; k < nx; k++) { cx[k] = std::cos(2 * M_PI * k / nx); sx[k] = -std::sin(2 * M_PI * k / nx); }
+  for (int k = 0; k < ny; k++) { cy[k] = std::cos(2 * M_PI * k / ny); sy[k] = -std::sin(2 * M_PI * k / ny); }
+  std::vector<double> tr((size_t)nx * ny), ti((size_t)nx * ny);
+  for (int j = 0; j < ny; j++)          // rows: transform along x
+    for (int u = 0; u < nx; u++) {
+      double ar = 0, ai = 0;
+      for (int i = 0; i < nx; i++) {
+        const int q = (int)(((long)u * i) % nx);
+        const double v = img[(size_t)j * nx + i];
+        ar += v * cx[q];
+        ai += v * sx[q];
+      }
+      tr[(size_t)j * nx + u] = ar;
+      ti[(size_t)j * nx + u] = ai;
+    }
+  re.assign((size_t)nx * ny, 0.0);
+  im.assign((size_t)nx * ny, 0.0);
+  for (int v = 0; v < ny; v++)          // columns: transform along y
+    for (int j = 0; j < ny; j++) {
+      const int q = (int)(((long)v * j) % ny);
+      const double c = cy[q], sn = sy[q];
+      const double* a = &tr[(size_t)j * nx];
+      const double* b = &ti[(size_t)j * nx];
+      double* orr = &re[(size_t)v * nx];
+      double* oi = &im[(size_t)v * nx];
+      for (int u = 0; u < nx; u++) {
+        orr[u] += a[u] * c - b[u] * sn;
+        oi[u] += a[u] * sn + b[u] * c;
+      }
+    }
+}
+
+extern "C" int icon_frc(const float* image1, const float* image2, int nx, int ny, float* frc) {
+  if (!image1 || !image2 || !frc || nx < 2 || ny < 1) {
+    set_err("icon_frc: bad argument");
+    return ICON_ERR_ARG;
+  }
+  std::vector<double> r1, i1, r2, i2;
+  dft2(image1, nx, ny, r1, i1);
+  dft2(image2, nx, ny, r2, i2);
+  const int h = nx / 2, hy = ny / 2;
+  std::vector<double> up(h + 1, 0.0), l1(h + 1, 0.0), l2(h + 1, 0.0);
+  const float damp2 = ((float)nx / (float)ny) * ((float)nx / (float)ny);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int i1c = (i + h) % nx, j1c = (j + hy) % ny;
+      // float arithmetic as the reference: int^2 + (float)(int^2) * damp2
+      const float q = (float)((i1c - h) * (i1c - h)) + (float)((j1c - hy) * (j1c - hy)) * damp2;
+      const int r = (int)std::floor(std::sqrt((double)q));
+      if (r < h) {
+        const size_t k = (size_t)j * nx + i;
+        up[r] += r1[k] * r2[k] + i1[k] * i2[k];
+        l1[r] += r1[k] * r1[k] + i1[k] * i1[k];
+        l2[r] += r2[k] * r2[k] + i2[k] * i2[k];
+      }
+    }
+  frc[0] = 1.f;
+  for (int r = 1; r < h; r++) {
+    const double d = std::sqrt(l1[r]) * std::sqrt(l2[r]);
+    frc[r] = d != 0 ? (float)(up[r] / d) : 0.f;
+  }
+  return ICON_OK;
+}
+
+// frcSmooth (ICONGPU.cu:59-91): running mean over a window of `overlap`
+// entries, shifted inwards at the ends.
+extern "C" void icon_frc_smooth(float* frc, int size, int overlap) {
+  std::vector<float> t(size);
+  for (int i = 0; i < size; i++) {
+    int jb = i - overlap / 2;
+    if (jb < 0) jb = 0;
+    int je = jb + overlap;
+    if (je > size - 1) {
+      jb -= je - size + 1;
+      je = size - 1;
+    }
+    if (jb < 0) jb = 0;
+    float sum = 0.f;
+    int num = 0;
+    for (int j = jb; j < je; j++) {
+      sum += frc[j];
+      num++;
+    }
+    t[i] = num > 0 ? sum / num : frc[i];
+  }
+  for (int i = 0; i < size; i++) frc[i] = t[i];
 }

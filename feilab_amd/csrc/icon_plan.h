@@ -18,8 +18,9 @@ constexpr int kMaxSlot = 3;    // node y-taps split over at most 3 column tiles 
 struct Geometry {
   int N0 = 0, N1 = 0, nz = 0, M = 0, n0 = 0, n1 = 0;
   float sig0 = 0, sig1 = 0, b0 = 0, b1 = 0, bf0 = 0, bf1 = 0;
-  int iomit = 0;
+  int iomit = 0;                  // tilt whose nodes give the reprojection
   float omit = 0;
+  int cv_skip = -1;               // cross-validation plan: input row left out of the data, else -1
   std::vector<float> theta;       // negated angles (MainCode.cu:42)
   std::vector<float> nodes;       // 2*M, (x, y) per node j = i*N0 + k
   std::vector<float> w;           // M density weights
@@ -47,7 +48,7 @@ struct Bins {
 };
 
 // Fills g; returns false (with a message) on an invalid geometry.
-bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err);
+bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv = false);
 void build_bins(const Geometry& g, int C, Bins& b);
 void grid_size(int N, float* sigma, int* n);
 

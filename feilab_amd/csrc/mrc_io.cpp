@@ -90,6 +90,45 @@ bool read_y_slices(FILE* fp, const MrcHeader& h, int y0, int count, float* out, 
   return true;
 }
 
+bool read_z_section(FILE* fp, const MrcHeader& h, int k, float* out, std::string* err) {
+  if (k < 0 || k >= h.nz) {
+    if (err) *err = "section outside the tilt series";
+    return false;
+  }
+  int64_t off = 0;
+  if (!data_offset(fp, h, &off, err)) return false;
+  const int es = element_size(h.mode);
+  const size_t n = (size_t)h.nx * h.ny;
+  std::vector<unsigned char> buf(n * es);
+  const int64_t pos = off + (int64_t)k * (int64_t)n * es;
+  if (fseeko(fp, (off_t)pos, SEEK_SET) != 0 || std::fread(buf.data(), 1, buf.size(), fp) != buf.size()) {
+    if (err) *err = "short read in MRC data";
+    return false;
+  }
+  const unsigned char* src = buf.data();
+  switch (h.mode) {
+    case kByte:
+      for (size_t i = 0; i < n; i++) out[i] = (float)src[i];
+      break;
+    case kShort:
+      for (size_t i = 0; i < n; i++) { int16_t v; std::memcpy(&v, src + 2 * i, 2); out[i] = (float)v; }
+      break;
+    case kUShort:
+      for (size_t i = 0; i < n; i++) { uint16_t v; std::memcpy(&v, src + 2 * i, 2); out[i] = (float)v; }
+      break;
+    default:
+      std::memcpy(out, src, n * 4);
+  }
+  return true;
+}
+
+bool write_real(const char* path, const MrcHeader& in, const float* data, int nx, int ny, std::string* err) {
+  MrcHeader h = in;
+  h.mz = in.mx;
+  h.zlen = in.xlen;
+  return write_float(path, h, data, nx, ny, 1, err);
+}
+
 bool write_float(const char* path, MrcHeader h, const float* data, int nx, int ny, int nz, std::string* err) {
   h.nx = nx; h.ny = ny; h.nz = nz; h.mode = kFloat;
   if (h.next < 0) h.next = 0;

@@ -56,6 +56,15 @@ void init_header(MrcHeader* h, int nx, int ny, int nz);
 // run of slices.  Returns false with *err set on failure.
 bool read_y_slices(FILE* fp, const MrcHeader& h, int y0, int count, float* out, std::string* err);
 
+// Section k (nx x ny, the z-axis slice of mrc_read_slice, mrcfile3.c:954-990)
+// as float.
+bool read_z_section(FILE* fp, const MrcHeader& h, int k, float* out, std::string* err);
+
+// saveMRC_real (saveMRC_real.cu:9-30) followed by mrc_update_head: header
+// copied from the tilt series with mz = mx, zlen = xlen, nx, ny, nz = 1, mode
+// float; data as given (x fastest).
+bool write_real(const char* path, const MrcHeader& in, const float* data, int nx, int ny, std::string* err);
+
 // Writes a float image nx (fast) x ny x nz with header `h` (nx, ny, nz, mode
 // overwritten), `h.next` zero bytes of extended header, then the data, and
 // fills amin/amax/amean/rms as mrc_update_head (mrcfile3.c:113-514) computes

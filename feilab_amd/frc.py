@@ -10,10 +10,14 @@ import numpy as np
 
 
 def _rings(nx, ny):
+    # the reference's float arithmetic: (float)(di^2) + (float)(dj^2) * damp2
+    # with damp2 = ((float)nx / (float)ny)^2 in float, sqrt in double
     i = (np.arange(nx) + nx // 2) % nx - nx // 2
     j = (np.arange(ny) + ny // 2) % ny - ny // 2
-    damp2 = (nx / ny) ** 2
-    r = np.floor(np.sqrt(i[None, :] ** 2 + (j[:, None] ** 2) * damp2)).astype(np.int64)
+    d = np.float32(np.float32(nx) / np.float32(ny))
+    damp2 = np.float32(d * d)
+    q = (i[None, :] ** 2).astype(np.float32) + (j[:, None] ** 2).astype(np.float32) * damp2
+    r = np.floor(np.sqrt(q.astype(np.float64))).astype(np.int64)
     return r
 
 

@@ -78,6 +78,7 @@ def lib():
         "icon_memset_d": [vp, ci, cs],
         "icon_device_synchronize": [],
         "icon_plan_create": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
+        "icon_plan_create_cv": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
         "icon_plan_destroy": [vp],
         "icon_plan_get_info": [vp, ctypes.POINTER(PlanInfo)],
         "icon_plan_set_col_block": [vp, ci],
@@ -91,6 +92,9 @@ def lib():
         "icon_reconstruct_host": [vp, ci, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolveOptions)],
         "icon_lib_2d2": [ci, ci, ci, ci, ctypes.c_char_p, vp, ctypes.c_char_p, ctypes.c_char_p,
                          ctypes.POINTER(ci), ctypes.POINTER(cf), ci],
+        "icon_lib_2d_crossv": [ci, ci, ci, ci, ctypes.c_char_p, vp, ctypes.c_char_p, ctypes.c_char_p,
+                               ctypes.POINTER(ci), ctypes.POINTER(cf), ci],
+        "icon_frc": [vp, vp, ci, ci, vp],
         "icon_read_tilts": [ctypes.c_char_p, ctypes.POINTER(cf), ci, ctypes.POINTER(ci)],
         "icon_partition_slices": [ci, ctypes.POINTER(ci), ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)],
         "icon_geometry_tables": [ci, ci, ci, ctypes.POINTER(cf), ctypes.POINTER(PlanInfo)] + [vp] * 10,
@@ -103,6 +107,8 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = ci
+    L.icon_frc_smooth.argtypes = [vp, ci, ci]
+    L.icon_frc_smooth.restype = None
     _lib = L
     return L
 
@@ -180,12 +186,16 @@ class Plan:
     node vectors [S][nz*nx] complex64, sinograms [S][nz][nx] float32.
     """
 
-    def __init__(self, nx, tilt_deg, thickness=0, max_slices=1, device=0):
+    def __init__(self, nx, tilt_deg, thickness=0, max_slices=1, device=0, cross_validation=False):
+        """cross_validation=True: the plan of Lib_2D_crossV2 (the least-|angle|
+        tilt left out of the data and used as the reprojection angle)."""
         tilt = np.ascontiguousarray(tilt_deg, np.float32)
         self.h = ctypes.c_void_p()
-        _check(lib().icon_plan_create(ctypes.byref(self.h), device, int(nx), int(thickness), len(tilt),
-                                      tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(max_slices)),
+        create = lib().icon_plan_create_cv if cross_validation else lib().icon_plan_create
+        _check(create(ctypes.byref(self.h), device, int(nx), int(thickness), len(tilt),
+                      tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(max_slices)),
                "icon_plan_create")
+        self.cross_validation = cross_validation
         info = PlanInfo()
         _check(lib().icon_plan_get_info(self.h, ctypes.byref(info)), "icon_plan_get_info")
         self.info = info
@@ -379,6 +389,44 @@ def partition_slices(cus, y_start, y_end):
     if used < 0 or used > nd:
         raise IconError(1, "partition failed")
     return [(f[i], l_[i]) for i in range(used)]
+
+
+def frc(image1, image2):
+    """calFRC_nonsqure (calFRC.cu:93-174) of two [ny][nx] images: nx/2 rings."""
+    a = np.ascontiguousarray(image1, np.float32)
+    b = np.ascontiguousarray(image2, np.float32)
+    ny, nx = a.shape
+    out = np.zeros(nx // 2 + 1, np.float32)
+    rc = lib().icon_frc(a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p), nx, ny,
+                        out.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise IconError(rc, lib().icon_lib_2d2_error().decode(errors="replace"))
+    return out[:nx // 2]
+
+
+def frc_smooth(curve, overlap=10):
+    """frcSmooth (ICONGPU.cu:59-91) of an FRC curve (returns a new array)."""
+    c = np.ascontiguousarray(curve, np.float32).copy()
+    lib().icon_frc_smooth(c.ctypes.data_as(ctypes.c_void_p), len(c), int(overlap))
+    return c
+
+
+def lib_2d_crossv(mrc_path, angle_path, result_dir, slice_begin, slice_end, pre, icon, post, data_type=1,
+                  threshold=0.03, thickness=0, device=0, batch=0):
+    """Lib_2D_crossV2 (MainCode_crossV.cu:3-189): the slices solved without the least-|angle| tilt;
+    returns the reprojections at that angle [slices][nx] (no files written)."""
+    import struct
+    with open(mrc_path, "rb") as fh:
+        nx = struct.unpack("<i", fh.read(4))[0]
+    rp = np.zeros((max(slice_end - slice_begin, 0), nx), np.float32)
+    before = (ctypes.c_int * 5)(pre, post, 1, data_type, 0)
+    betain = (ctypes.c_float * 2)(threshold, float(thickness))
+    rc = lib().icon_lib_2d_crossv(device, slice_begin, slice_end, pre + icon + post, mrc_path.encode(),
+                                  rp.ctypes.data_as(ctypes.c_void_p), angle_path.encode(), result_dir.encode(),
+                                  before, betain, batch)
+    if rc:
+        raise IconError(rc, lib().icon_lib_2d2_error().decode(errors="replace"))
+    return rp
 
 
 def lib_2d2(mrc_path, angle_path, result_dir, slice_begin, slice_end, pre, icon, post, data_type=1,

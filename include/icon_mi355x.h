@@ -90,6 +90,14 @@ int icon_device_synchronize(void);
  * initGICONPara2. */
 int icon_plan_create(icon_plan** plan, int device, int nx, int thickness, int nz,
                      const float* tilt_deg, int max_slices);
+/* Cross-validation plan (Lib_2D_crossV2, MainCode_crossV.cu:3-189): the
+ * first tilt of least |angle| is left out of the data (its row of the
+ * sinograms is skipped; the other nz-1 tilts get their own density weights)
+ * and the reconstruction is reprojected at that angle.  Takes the same
+ * [S][nz][N0] sinograms as the full plan; icon_reconstruct's d_reproj then
+ * holds the cross-validation reprojections. */
+int icon_plan_create_cv(icon_plan** plan, int device, int nx, int thickness, int nz,
+                        const float* tilt_deg, int max_slices);
 int icon_plan_destroy(icon_plan* plan);
 int icon_plan_get_info(const icon_plan* plan, icon_plan_info* info);
 /* Tune the spread/interp column tile (8 or 16 grid columns); rebuilds the
@@ -186,7 +194,20 @@ int icon_copy_probe(void* d_dst, const void* d_src, size_t bytes, int reps, doub
 int icon_lib_2d2(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
                  float* reproj, const char* angle_path, const char* result_dir, const int* before,
                  const float* betain, int batch);
-/* Last failure of icon_lib_2d2 on any thread. */
+/* Lib_2D_crossV2 (MainCode_crossV.cu:3-189): the same slices solved without
+ * the least-|angle| tilt (icon_plan_create_cv), reprojected at that angle
+ * into `reproj`; no slice files are written. */
+int icon_lib_2d_crossv(int device, int slice_begin, int slice_end, int iternum, const char* mrc_path,
+                       float* reproj, const char* angle_path, const char* result_dir, const int* before,
+                       const float* betain, int batch);
+/* calFRC_nonsqure (calFRC.cu:93-174): Fourier ring correlation of two
+ * nx (fast) x ny images, rings floor(sqrt(di^2 + (dj nx/ny)^2)) < nx/2;
+ * frc[0..nx/2-1], frc[0] = 1, 0 for an empty ring.  Host computation. */
+int icon_frc(const float* image1, const float* image2, int nx, int ny, float* frc);
+/* frcSmooth (ICONGPU.cu:59-91): in-place running mean over `overlap`
+ * entries, the window shifted inwards at the ends. */
+void icon_frc_smooth(float* frc, int size, int overlap);
+/* Last failure of icon_lib_2d2 / icon_lib_2d_crossv on any thread. */
 const char* icon_lib_2d2_error(void);
 
 /* Tilt file reader (MainCode.cu:38-45): whitespace-separated degrees, as

@@ -10,6 +10,8 @@ CPU restatement of the reference on the same seeded inputs.  Tolerances:
   (the bar for "the reference's own reconstruction"), data residual within
   1e-4 relative.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -211,3 +213,72 @@ def test_warm_start_continues(icon, oracle_mod):
     m = plan.solve(s0, 0, 10, 2, 2, 0.0, m_init=m_init)
     mo = orc.solve(s0, 0, 10, 2, 2, 0.0, m_init=m_init)
     assert rel(m, mo) < 1e-4
+
+
+def test_cross_validation_plan_vs_oracle(icon, oracle_mod):
+    """icon_plan_create_cv is the solve without the least-|angle| tilt (the
+    other tilts keep their order and get their own weights) reprojected at that
+    angle (MainCode_crossV.cu:45-60,113-160): GPU vs the FP32-mirror oracle run
+    on the reduced tilt series, on an asymmetric range so that the left-out
+    tilt is not the middle one."""
+    _print("cross validation plan")
+    tilt, sino = _small_case(17)
+    tilt, sino = tilt[3:], sino[3:]
+    N = sino.shape[1]
+    plan = icon.Plan(N, tilt, max_slices=1, cross_validation=True)
+    assert plan.info.iomit == len(tilt) - 1
+    m, rp, res, al = plan.solve(sino, 4, 20, 4, 2, 0.0, trace=True, reproject=True)
+    k = int(np.argmin(np.abs(tilt)))
+    orc = oracle_mod.Plan(N, N, np.delete(tilt, k), 32)
+    mo, reso, alo = orc.solve(np.delete(sino, k, axis=0), 4, 20, 4, 2, 0.0, trace=True)
+    assert rel(m, mo) < 1e-4
+    assert np.allclose(res, reso, rtol=1e-4, atol=1e-7)
+    assert np.allclose(al, alo, rtol=1e-4)
+    rpo = oracle_mod.Plan(N, N, tilt, 32).reproject(mo)
+    assert rel(rp, rpo) < 1e-4
+    # the full plan's reprojection of the same image is the same line
+    full = icon.Plan(N, tilt, max_slices=1)
+    assert full.info.iomit == k
+
+
+def test_cli_end_to_end_with_cross_validation(icon, oracle_mod, tmp_path):
+    """icon-gpu on an MRC tilt series: one midYYYYY.mrc per slice equal to the
+    oracle's solve, and the cross-validation outputs of ICONGPU.cu:468-592."""
+    import subprocess
+    from feilab_amd import mrc
+    _print("cli end to end")
+    rng = np.random.default_rng(23)
+    N, S = 64, 5
+    tilt = phantom.tilt_series(60, 6)
+    b = phantom.blobs_3d(N, S, N, 25, rng)
+    sinos = phantom.add_noise(phantom.project_3d(b, N, np.arange(S), tilt), 1.0, rng)
+    mrc.write_tilt_series(str(tmp_path / "ts.mrc"), sinos)
+    (tmp_path / "ts.tlt").write_text("\n".join("%.2f" % t for t in tilt) + "\n")
+    out = tmp_path / "out"
+    out.mkdir()
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "feilab_amd", "icon-gpu")
+    r = subprocess.run([exe, "-i", str(tmp_path / "ts.mrc"), "-t", str(tmp_path / "ts.tlt"), "-o", str(out),
+                        "-s", "0,%d" % (S - 1), "-iter", "2,6,2", "-d", "2", "-thr", "0", "-g", "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    orc = oracle_mod.Plan(N, N, tilt, 32)
+    for s in range(S):
+        _, vol = mrc.read(str(out / "reconstruction" / ("mid%05d.mrc" % s)))
+        mo = orc.solve(sinos[s], 2, 6, 2, 2, 0.0)
+        assert rel(vol[0].T, mo) < 1e-4, s
+    cv = out / "crossValidation"
+    k = int(np.argmin(np.abs(tilt)))
+    _, gt = mrc.read(str(cv / "GroundTruth.mrc"))
+    np.testing.assert_array_equal(gt[0], sinos[:, k, :])
+    _, full = mrc.read(str(cv / "fullRec_reProjection.mrc"))
+    _, cvr = mrc.read(str(cv / "crossV_reprojection.mrc"))
+    for name, rec in (("fullRec.frc", full[0]), ("crossV.frc", cvr[0])):
+        rows = np.loadtxt(str(cv / name))
+        assert rows.shape == (N // 2, 2)
+        np.testing.assert_allclose(rows[:, 0], np.arange(N // 2) / N, atol=1e-6)
+        expect = icon.icon.frc_smooth(icon.icon.frc(gt[0], rec), 10)
+        np.testing.assert_allclose(rows[:, 1], expect, atol=2e-6)
+    # the cross-validation reprojection of a slice = the CV plan's, from the oracle
+    orc_cv = oracle_mod.Plan(N, N, np.delete(tilt, k), 32)
+    mo = orc_cv.solve(np.delete(sinos[1], k, axis=0), 2, 6, 2, 2, 0.0)
+    assert rel(cvr[0][1], orc.reproject(mo)) < 1e-4

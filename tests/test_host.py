@@ -167,3 +167,29 @@ def test_cli_without_device_exits_cleanly(tmp_path):
     r = subprocess.run([_cli(), "-i", "x.mrc", "-t", "x.tlt", "-o", str(tmp_path), "-s", "0,3", "-iter", "1,1,1"],
                        capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0 and "No GPU is specified to use" in r.stdout
+
+
+def test_frc_matches_independent_fft_implementation(icon):
+    """icon_frc (host DFT) against the numpy-FFT FRC in feilab_amd.frc, on a
+    non-square pair, plus the closed forms frc(a, a) = 1 and frc(a, -a) = -1."""
+    from feilab_amd import frc as nfrc
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((24, 40)).astype(np.float32)
+    b = (a + 0.7 * rng.standard_normal((24, 40))).astype(np.float32)
+    c = icon.icon.frc(a, b)
+    assert c.shape == (20,)
+    np.testing.assert_allclose(c, nfrc.frc(a, b), atol=2e-6)
+    np.testing.assert_allclose(icon.icon.frc(a, a), 1.0, atol=1e-6)
+    np.testing.assert_allclose(icon.icon.frc(a, -a)[1:], -1.0, atol=1e-6)
+
+
+def test_frc_smooth_window():
+    """frcSmooth: a window of `overlap` entries starting overlap/2 before i,
+    shifted inwards at the ends (the last entry is never averaged in)."""
+    import feilab_amd
+    x = np.arange(8, dtype=np.float32)
+    s = feilab_amd.icon.frc_smooth(x, 4)
+    # i=0: [0,4) -> 1.5; i=1: [0,4); i=2: [0,4); i=3: [1,5) -> 2.5; i=4: [2,6) -> 3.5;
+    # i=5: [3,7) -> 4.5; i=6: [4,8) clipped to [3,7) -> 4.5; i=7: same
+    np.testing.assert_allclose(s, [1.5, 1.5, 1.5, 2.5, 3.5, 4.5, 4.5, 4.5])
+    np.testing.assert_allclose(feilab_amd.icon.frc_smooth(np.full(30, 0.25, np.float32), 10), 0.25)
