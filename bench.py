@@ -77,6 +77,77 @@ def cpu_baseline(cfg, S_cpu, iters_cpu, total_iters, threads):
                        % (S_cpu, iters_cpu, total_iters, len(cfg["tilt"]), iters_cpu, total_iters, dt))
 
 
+COPY_BYTES = 1 << 30
+
+
+def probe(what, S):
+    """Child-process mode for the counter passes: run the named ICON kernel on
+    the workload's buffers (or a 1 GiB plain copy) a few times and exit."""
+    import feilab_amd
+    from feilab_amd import phantom
+    from feilab_amd.icon import DeviceArray, SolveOptions, copy_probe
+    if what == "copy":
+        copy_probe(COPY_BYTES, 2)
+        return
+    cfg = phantom.config2(nslices=S)
+    plan = feilab_amd.Plan(cfg["N"], cfg["tilt"], max_slices=S, device=0)
+    d_s = DeviceArray.from_host(cfg["sino"], np.float32)
+    d_m = DeviceArray((S, plan.N0, plan.N1), np.float32)
+    plan.reconstruct_dev(S, d_s, d_m, SolveOptions(1, 1, 0, 2, 0.0, 0, 0))
+    feilab_amd.lib().icon_device_synchronize()
+    plan.time_kernel(what, S, 2)
+
+
+def counter_pass(counter, target, S, kernel_substr):
+    """Average per-dispatch value (KiB) of one TCC counter for the dispatches
+    whose name contains kernel_substr, from a rocprofv3 --pmc child run of the
+    probe mode; None if the profiler is unavailable or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    d = tempfile.mkdtemp(prefix="bench_pmc_")
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--",
+           sys.executable, os.path.abspath(__file__), "--probe", target, "--slices", str(S)]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120,
+                           env=dict(os.environ, TMPDIR=d))
+    except subprocess.TimeoutExpired:
+        log("counter pass %s on %s timed out" % (counter, target))
+        return None
+    vals = {}
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                for row in csv.DictReader(open(os.path.join(root, f))):
+                    if kernel_substr in row["Kernel_Name"] and row["Counter_Name"] == counter:
+                        vals[row["Dispatch_Id"]] = vals.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+    shutil.rmtree(d, ignore_errors=True)
+    if r.returncode != 0 and not vals:
+        log("counter pass %s on %s failed (rc %d)" % (counter, target, r.returncode))
+        return None
+    return float(np.mean(list(vals.values()))) if vals else None
+
+
+def measure_traffic(kernel, S):
+    """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE, each
+    scaled by what the same counter reads for a plain copy of known size."""
+    name = {"col_fwd": "k_col_fwd", "col_adj": "k_col_adj"}[kernel]
+    f_k = counter_pass("FETCH_SIZE", kernel, S, name)
+    w_k = counter_pass("WRITE_SIZE", kernel, S, name)
+    f_c = counter_pass("FETCH_SIZE", "copy", S, "k_copy_f4")
+    w_c = counter_pass("WRITE_SIZE", "copy", S, "k_copy_f4")
+    if None in (f_k, w_k, f_c, w_c) or f_c <= 0 or w_c <= 0:
+        return None, None
+    fs, ws = COPY_BYTES / (f_c * 1024), COPY_BYTES / (w_c * 1024)
+    traffic = f_k * 1024 * fs + w_k * 1024 * ws
+    return traffic, dict(fetch_kib=round(f_k, 1), write_kib=round(w_k, 1), copy_fetch_kib=round(f_c, 1),
+                         copy_write_kib=round(w_c, 1), fetch_scale=round(fs, 4), write_scale=round(ws, 4))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +168,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--col-block", type=int, default=0, help="grid columns per spread/interp tile (0: default)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--probe", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.probe:
+        probe(args.probe, args.slices)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the ICON solve shards by slice with no exchange between devices, "
               "and this build runs on a single MI355X; --gpus %d is not supported." % args.gpus)
@@ -164,6 +240,14 @@ def main():
     roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
                     algo_bytes="%d per slice x %d slices per launch" % (kb[dom], S))
+    if not args.no_traffic:
+        log("counter passes (FETCH_SIZE, WRITE_SIZE) on %s and a 1 GiB copy" % dom)
+        traffic, detail = measure_traffic(dom, S)
+        if traffic is not None:
+            roofline["traffic"] = int(traffic)
+            roofline["traffic_detail"] = detail
+            roofline["algo_bytes_per_launch"] = kb[dom] * S
+        log("traffic:", roofline["traffic"])
 
     # ---- learned leg
     learned = None
