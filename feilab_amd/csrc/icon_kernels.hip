@@ -169,24 +169,24 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int h0 = a.N0 >> 1;
-  // load the non-zero rows (zero padding elsewhere): every load in flight
-  // before the first LDS store
-  constexpr int LPT = NC * C / NT;   // 16
-  float2 ld[LPT];
+  // x FFT of column c = tid % C: the first pass takes its 16 inputs (grid
+  // rows t + r NC/16; zero padding outside the N0 non-zero rows) straight
+  // from global memory, all loads in flight together; the transform leaves
+  // its output in the LDS tile for the interpolation
+  {
+    const int c = tid % C, t = tid / C;
+    float2 v[16];
 #pragma unroll
-  for (int k = 0; k < LPT; k++) {
-    const int i = tid + k * NT;
-    const int x = i / C, c = i % C;
-    const int X = (x + h0) & (NC - 1);
-    ld[k] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
-  }
+    for (int r = 0; r < 16; r++) {
+      const int x = in_index<NC>(t, r);
+      const int X = (x + h0) & (NC - 1);
+      v[r] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
+    }
+    fft_regs<NC, LS, -1>(v, tile + c, t, a.tw);
+    // the last exchange ended with a barrier: the column can be overwritten
 #pragma unroll
-  for (int k = 0; k < LPT; k++) {
-    const int i = tid + k * NT;
-    tile[(i / C) * LS + i % C] = ld[k];
+    for (int i = 0; i < 16; i++) tile[out_index<NC>(t, i) * LS + c] = v[i];
   }
-  __syncthreads();
-  fft_lds<NC, LS, -1>(tile + (tid % C), tid / C, a.tw);
   __syncthreads();
   // interpolation partials of this tile's nodes
   const int lane = tid & 15;
@@ -374,14 +374,20 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
     }
   }
   __syncthreads();
-  fft_lds<NC, LS, 1>(tile + (tid % C), tid / C, a.tw);
-  __syncthreads();
-  // keep the N0 rows the crop needs
-  const int h0 = a.N0 >> 1;
-  for (int i = tid; i < a.N0 * C; i += NT) {
-    const int X = i / C, c = i % C;
-    const int x = (X - h0) & (NC - 1);
-    a.out[((size_t)s * a.N0 + X) * a.n1 + y0 + c] = tile[x * LS + c];
+  // inverse x FFT of column c = tid % C; the last pass writes the N0 rows the
+  // crop keeps (grid row x -> image row X = x + N0/2 mod NC) straight to global
+  {
+    const int c = tid % C, t = tid / C;
+    float2 v[16];
+    pass_load<NC, 16, LS>(v, tile + c, t);
+    __syncthreads();
+    fft_regs<NC, LS, 1>(v, tile + c, t, a.tw);
+    const int h0 = a.N0 >> 1;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int X = (out_index<NC>(t, i) + h0) & (NC - 1);
+      if (X < a.N0) a.out[((size_t)s * a.N0 + X) * a.n1 + y0 + c] = v[i];
+    }
   }
 }
 
