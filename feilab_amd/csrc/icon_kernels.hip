@@ -144,9 +144,8 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
 // One block per (slice, tile of C grid columns); the n0 x C tile lives in LDS,
 // row-major with a padded row stride of C+1 float2 so that rows map to
 // different banks.  FFT: thread (c = tid % C, t = tid / C) works on column c.
-// Interpolation: 16-lane groups take one bin entry (node, run of y-taps
-// inside the tile) each; lane tx < 14 reads the C cells of grid row
-// x0 + tx and the group sums its 14 rows with a 16-lane shuffle reduction.
+// Interpolation: one bin entry (node, run of y-taps inside the tile) per
+// thread, summing its 14 x C cells of the tile.
 // Spread: see k_col_adj (row-owner gather, no atomics).
 
 // Row stride (float2) of the LDS tile: C + 2 keeps rows 16-byte aligned for
@@ -160,7 +159,6 @@ template <int NC, int C>
 __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   constexpr int NT = C * (NC / 16);
   constexpr int LS = col_stride<NC, C>();
-  constexpr int NG = NT / 16;
   extern __shared__ __align__(16) float2 tile[];   // NC * LS
   // tile-major order: blocks in flight together work on the same column
   // tile of different slices and share its bin records in L2
@@ -188,14 +186,20 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
     for (int i = 0; i < 16; i++) tile[out_index<NC>(t, i) * LS + c] = v[i];
   }
   __syncthreads();
-  // interpolation partials of this tile's nodes
-  const int lane = tid & 15;
-  const int grp = tid >> 4;
+  // interpolation partials of this tile's nodes: one bin entry per thread;
+  // the entry's record (first x-tap, 14 x-taps, C y-taps) comes in with
+  // vector loads and the thread sums its 14 x C cells from the LDS tile
   const int e1 = a.ent_off[blk + 1];
-  auto interp = [&](int e) {
+  for (int e = a.ent_off[blk] + tid; e < e1; e += NT) {
     const int4 en = a.ent[e];
-    // lane q < 14: the node's x-tap q; every lane: the y-taps of the C columns
-    const float px = lane < kTapsK ? a.ent_px[(size_t)e * kTapsK + lane] : 0.f;
+    float px[kTapsK];
+    const float2* pxv = reinterpret_cast<const float2*>(a.ent_px + (size_t)e * kTapsK);
+#pragma unroll
+    for (int q = 0; q < kTapsK / 2; q++) {
+      const float2 p = pxv[q];
+      px[2 * q] = p.x;
+      px[2 * q + 1] = p.y;
+    }
     float py[C];
     const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
 #pragma unroll
@@ -204,16 +208,16 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
       py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
     }
     float ax = 0.f, ay = 0.f;
-    if (lane < kTapsK) {
-      const float2* row = tile + ((en.z + lane) & (NC - 1)) * LS;
+#pragma unroll
+    for (int q = 0; q < kTapsK; q++) {
+      const float2* row = tile + ((en.z + q) & (NC - 1)) * LS;
       float2 gv[C];
       if constexpr (LS % 2 == 0) {
-        // 16-byte aligned rows: two cells per LDS read
 #pragma unroll
         for (int c2 = 0; c2 < C / 2; c2++) {
-          const float4 q = reinterpret_cast<const float4*>(row)[c2];
-          gv[2 * c2] = make_float2(q.x, q.y);
-          gv[2 * c2 + 1] = make_float2(q.z, q.w);
+          const float4 g4 = reinterpret_cast<const float4*>(row)[c2];
+          gv[2 * c2] = make_float2(g4.x, g4.y);
+          gv[2 * c2 + 1] = make_float2(g4.z, g4.w);
         }
       } else {
 #pragma unroll
@@ -221,25 +225,13 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
       }
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        const float phi = px * py[c];
+        const float phi = px[q] * py[c];
         ax = __fmaf_rn(gv[c].x, phi, ax);
         ay = __fmaf_rn(gv[c].y, phi, ay);
       }
     }
-#pragma unroll
-    for (int o = 8; o >= 1; o >>= 1) {
-      ax += __shfl_xor(ax, o, 16);
-      ay += __shfl_xor(ay, o, 16);
-    }
-    if (lane == 0) a.part[((size_t)s * a.M + en.x) * kMaxSlotK + ((en.y >> 16) & 0xff)] = make_float2(ax, ay);
-  };
-  // two entries per group per iteration: their loads are in flight together
-  int e = a.ent_off[blk] + grp;
-  for (; e + NG < e1; e += 2 * NG) {
-    interp(e);
-    interp(e + NG);
+    a.part[((size_t)s * a.M + en.x) * kMaxSlotK + ((en.y >> 16) & 0xff)] = make_float2(ax, ay);
   }
-  if (e < e1) interp(e);
 }
 
 // Wavefront-scope barrier for LDS data exchanged between the lanes of one wave.
@@ -297,21 +289,21 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
     } else {
       lo1 = xo[NC - (kTapsK - 1) + xa]; hi1 = xo[NC]; lo2 = xo[0]; hi2 = xo[xa + 64];
     }
-    // this lane's 16-lane group owns rows [xg, xg+16): its candidates are the
-    // entries with first x-tap in [xg-13, xg+15], per run an index range
-    const int xg = xa + (lane & ~15);
-    int glo[2], ghi[2];
-    if (xg >= kTapsK - 1) {
-      glo[0] = ghi[0] = 0;
-      glo[1] = xo[xg - (kTapsK - 1)]; ghi[1] = xo[xg + 16];
+    // this lane's own candidates: the entries with first x-tap in [x-13, x]
+    // (mod NC), as an index range in each of the wave's runs; every one of
+    // them covers row x
+    int llo[2], lhi[2];
+    if (x >= kTapsK - 1) {
+      llo[0] = lhi[0] = 0;
+      llo[1] = xo[x - (kTapsK - 1)]; lhi[1] = xo[x + 1];
     } else {
-      glo[0] = xo[NC - (kTapsK - 1) + xg]; ghi[0] = xo[NC];
-      glo[1] = xo[0]; ghi[1] = xo[xg + 16];
+      llo[0] = xo[NC - (kTapsK - 1) + x]; lhi[0] = xo[NC];
+      llo[1] = xo[0]; lhi[1] = xo[x + 1];
     }
     for (int run = 0; run < 2; run++) {
       const int lo = run ? lo2 : lo1, hi = run ? hi2 : hi1;
       // the wrap run of the first wave is run 0; otherwise the single run is run 1
-      const int gr = (xa >= kTapsK - 1) ? 1 : run;
+      const int lr = (xa >= kTapsK - 1) ? 1 : run;
       for (int base = lo; base < hi; base += CH) {
         const int cnt = min(CH, hi - base);
         wave_sync();
@@ -332,35 +324,27 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
           rec[4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
         }
         wave_sync();
-        // the group's part of this chunk
-        const int k0 = max(glo[gr], base) - base;
-        const int k1 = min(ghi[gr], base + cnt) - base;
-        auto add = [&](int k) {
+        // this lane's part of the chunk, in list order
+        const int k0 = max(llo[lr], base) - base;
+        const int k1 = min(lhi[lr], base + cnt) - base;
+        for (int k = k0; k < k1; k++) {
           const float* rec = stage + k * REC;
           const float4 tail = reinterpret_cast<const float4*>(rec)[4 + C / 4];
-          const int tx = (x - __float_as_int(tail.z)) & (NC - 1);
-          if (tx < kTapsK) {
-            const float px = rec[tx];
-            float py[C];
+          const int tx = (x - __float_as_int(tail.z)) & (NC - 1);   // < 14 by the range
+          const float px = rec[tx];
+          float py[C];
 #pragma unroll
-            for (int c4 = 0; c4 < C / 4; c4++) {
-              const float4 v = reinterpret_cast<const float4*>(rec)[4 + c4];
-              py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
-            }
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-              const float phi = px * py[c];
-              acc[r][c].x = __fmaf_rn(tail.x, phi, acc[r][c].x);
-              acc[r][c].y = __fmaf_rn(tail.y, phi, acc[r][c].y);
-            }
+          for (int c4 = 0; c4 < C / 4; c4++) {
+            const float4 v = reinterpret_cast<const float4*>(rec)[4 + c4];
+            py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
           }
-        };
-        int k = k0;
-        for (; k + 1 < k1; k += 2) {
-          add(k);
-          add(k + 1);
+#pragma unroll
+          for (int c = 0; c < C; c++) {
+            const float phi = px * py[c];
+            acc[r][c].x = __fmaf_rn(tail.x, phi, acc[r][c].x);
+            acc[r][c].y = __fmaf_rn(tail.y, phi, acc[r][c].y);
+          }
         }
-        if (k < k1) add(k);
       }
     }
   }
@@ -539,7 +523,6 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
 
 template <int NR>
 __global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
-  constexpr int T = NR / 16;
   __shared__ float2 lds[NR];
   const int s = blockIdx.x;
   const int t = threadIdx.x;
