@@ -121,23 +121,51 @@ __device__ __forceinline__ void pass_load(float2 (&v)[16], const float2* lds, in
     for (int r = 0; r < RP; r++) v[q * RP + r] = lds[(t + q * T + r * (N / RP)) * ES];
 }
 
-// Twiddle and DFT of a pass with radix RP whose sub-transforms have size NS.
-template <int N, int RP, int NS, int DIR>
-__device__ __forceinline__ void pass_compute(float2 (&v)[16], int t, const float2* __restrict__ tw) {
-  constexpr int T = N / 16;
-  constexpr int B = 16 / RP;
+// Twiddles of the passes after the first (at most two for N <= 4096), per
+// pass the B groups x (RP - 1) factors this thread applies.  They depend only
+// on the thread's index, so they are loaded up front and their latency
+// overlaps the data loads and the first pass.
+struct FftTwiddles {
+  float2 w[2][16];
+};
+
+template <int N, int NS, int P, int DIR>
+__device__ __forceinline__ void load_twiddles_from(FftTwiddles& T, int t, const float2* __restrict__ tw) {
+  constexpr int RP = pass_radix<N, NS>();
+  if constexpr (NS > 1) {
+    static_assert(P < 2, "at most two twiddled passes");
+    constexpr int TH = N / 16;
+    constexpr int B = 16 / RP;
+    constexpr int step = N / (NS * RP);
 #pragma unroll
-  for (int q = 0; q < B; q++) {
-    if constexpr (NS > 1) {
-      const int j = t + q * T;
-      const int k = j & (NS - 1);
-      constexpr int step = N / (NS * RP);
+    for (int q = 0; q < B; q++) {
+      const int k = (t + q * TH) & (NS - 1);
 #pragma unroll
       for (int r = 1; r < RP; r++) {
         float2 w = tw[r * k * step];
         if (DIR > 0) w.y = -w.y;
-        v[q * RP + r] = c_mul(v[q * RP + r], w);
+        T.w[P][q * (RP - 1) + r - 1] = w;
       }
+    }
+  }
+  if constexpr (NS * RP < N) load_twiddles_from<N, NS * RP, (NS > 1 ? P + 1 : 0), DIR>(T, t, tw);
+}
+
+template <int N, int DIR>
+__device__ __forceinline__ void load_twiddles(FftTwiddles& T, int t, const float2* __restrict__ tw) {
+  load_twiddles_from<N, 1, 0, DIR>(T, t, tw);
+}
+
+// Twiddle and DFT of a pass with radix RP whose sub-transforms have size NS
+// (twiddle slot P of T when NS > 1).
+template <int N, int RP, int NS, int DIR, int P>
+__device__ __forceinline__ void pass_compute(float2 (&v)[16], const FftTwiddles& T) {
+  constexpr int B = 16 / RP;
+#pragma unroll
+  for (int q = 0; q < B; q++) {
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < RP; r++) v[q * RP + r] = c_mul(v[q * RP + r], T.w[P][q * (RP - 1) + r - 1]);
     }
     dft<RP, DIR>(&v[q * RP]);
   }
@@ -173,25 +201,34 @@ __device__ __forceinline__ void line_sync() {
 
 // Runs the passes from sub-transform size NS on; v holds that pass's inputs.
 // Every thread of the block (WAVE = 0) or of the wave (WAVE = 1) must call it.
-template <int N, int ES, int DIR, int NS, int WAVE = 0>
-__device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, const float2* __restrict__ tw) {
+template <int N, int ES, int DIR, int NS, int WAVE = 0, int P = 0>
+__device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {
   constexpr int RP = pass_radix<N, NS>();
-  pass_compute<N, RP, NS, DIR>(v, t, tw);
+  pass_compute<N, RP, NS, DIR, P>(v, T);
   if constexpr (NS * RP < N) {
     pass_store<N, RP, NS, ES>(v, lds, t);
     line_sync<WAVE>();
     constexpr int RN = pass_radix<N, NS * RP>();
     pass_load<N, RN, ES>(v, lds, t);
     line_sync<WAVE>();
-    fft_from<N, ES, DIR, NS * RP, WAVE>(v, lds, t, tw);
+    fft_from<N, ES, DIR, NS * RP, WAVE, (NS > 1 ? P + 1 : 0)>(v, lds, t, T);
   }
 }
 
 // Registers in (input order) -> registers out (output order).  lds: N
-// float2 of scratch for this line at element stride ES.
+// float2 of scratch for this line at element stride ES; T from
+// load_twiddles<N, DIR>.
+template <int N, int ES, int DIR>
+__device__ __forceinline__ void fft_regs(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {
+  fft_from<N, ES, DIR, 1>(v, lds, t, T);
+}
+
+// The same, loading the twiddles itself.
 template <int N, int ES, int DIR>
 __device__ __forceinline__ void fft_regs(float2 (&v)[16], float2* lds, int t, const float2* __restrict__ tw) {
-  fft_from<N, ES, DIR, 1>(v, lds, t, tw);
+  FftTwiddles T;
+  load_twiddles<N, DIR>(T, t, tw);
+  fft_from<N, ES, DIR, 1>(v, lds, t, T);
 }
 
 // Element index held in output slot i of thread t.
@@ -212,10 +249,12 @@ __device__ __forceinline__ int in_index(int t, int r) {
 // Callers synchronise before (data present) and after (data consumed).
 template <int N, int ES, int DIR>
 __device__ __forceinline__ void fft_lds(float2* lds, int t, const float2* __restrict__ tw) {
+  FftTwiddles T;
+  load_twiddles<N, DIR>(T, t, tw);
   float2 v[16];
   pass_load<N, 16, ES>(v, lds, t);
   __syncthreads();
-  fft_regs<N, ES, DIR>(v, lds, t, tw);
+  fft_regs<N, ES, DIR>(v, lds, t, T);
   // the last pass_load is followed by a barrier inside fft_from, so the
   // line can be overwritten now
 #pragma unroll
@@ -228,10 +267,12 @@ __device__ __forceinline__ void fft_lds(float2* lds, int t, const float2* __rest
 template <int N, int ES, int DIR>
 __device__ __forceinline__ void fft_lds_wave(float2* lds, int t, const float2* __restrict__ tw) {
   static_assert(N / 16 == 64, "one wavefront per line");
+  FftTwiddles T;
+  load_twiddles<N, DIR>(T, t, tw);
   float2 v[16];
   pass_load<N, 16, ES>(v, lds, t);
   line_sync<1>();
-  fft_from<N, ES, DIR, 1, 1>(v, lds, t, tw);
+  fft_from<N, ES, DIR, 1, 1>(v, lds, t, T);
 #pragma unroll
   for (int i = 0; i < 16; i++) lds[out_index<N>(t, i) * ES] = v[i];
 }

@@ -51,6 +51,8 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
   const int s = valid ? row / a.N0 : 0;
   const int X = valid ? row % a.N0 : 0;
   const int h1 = a.N1 >> 1;
+  FftTwiddles Tw;
+  load_twiddles<NR, -1>(Tw, t, a.tw);
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
     }
     v[r] = gv;
   }
-  fft_regs<NR, 1, -1>(v, lds + team * NR, t, a.tw);
+  fft_regs<NR, 1, -1>(v, lds + team * NR, t, Tw);
   if (valid) {
     float2* o = a.out + (size_t)row * NR;
 #pragma unroll
@@ -102,11 +104,13 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
   const bool valid = row < a.rows;
   const int s = valid ? row / a.N0 : 0;
   const int X = valid ? row % a.N0 : 0;
+  FftTwiddles Tw;
+  load_twiddles<NR, 1>(Tw, t, a.tw);
   float2 v[16];
   const float2* src = a.in + (size_t)(valid ? row : 0) * NR;
 #pragma unroll
   for (int r = 0; r < 16; r++) v[r] = valid ? src[in_index<NR>(t, r)] : make_float2(0.f, 0.f);
-  fft_regs<NR, 1, 1>(v, lds + team * NR, t, a.tw);
+  fft_regs<NR, 1, 1>(v, lds + team * NR, t, Tw);
   const int h1 = a.N1 >> 1;
   double acc = 0.0;
   if (valid) {
@@ -173,6 +177,8 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
   // its output in the LDS tile for the interpolation
   {
     const int c = tid % C, t = tid / C;
+    FftTwiddles T;
+    load_twiddles<NC, -1>(T, t, a.tw);
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -180,7 +186,7 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
       const int X = (x + h0) & (NC - 1);
       v[r] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
     }
-    fft_regs<NC, LS, -1>(v, tile + c, t, a.tw);
+    fft_regs<NC, LS, -1>(v, tile + c, t, T);
     // the last exchange ended with a barrier: the column can be overwritten
 #pragma unroll
     for (int i = 0; i < 16; i++) tile[out_index<NC>(t, i) * LS + c] = v[i];
@@ -348,6 +354,9 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
       }
     }
   }
+  // twiddles of the inverse x FFT, in flight while the tile is written
+  FftTwiddles Tw;
+  load_twiddles<NC, 1>(Tw, tid / C, a.tw);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
@@ -365,7 +374,7 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
     float2 v[16];
     pass_load<NC, 16, LS>(v, tile + c, t);
     __syncthreads();
-    fft_regs<NC, LS, 1>(v, tile + c, t, a.tw);
+    fft_regs<NC, LS, 1>(v, tile + c, t, Tw);
     const int h0 = a.N0 >> 1;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
