@@ -125,19 +125,48 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs<TX, TY> a) {
       *reinterpret_cast<uint4*>(&s_w[r * kLdsRow + 8 * c]) = v;
     }
     __syncthreads();
-    for (int t = 0; t < taps; t++) {
-      const int hy = taps == 9 ? t / 3 : 1, hx = taps == 9 ? t % 3 : 1;   // halo offsets
+    if (taps == 9) {
+      // for a column offset dx the A operand of tile row i at row offset dy is
+      // halo row 4w + i + dy: six halo rows serve all three dy
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++) {
+        bf8 A[6];
+#pragma unroll
+        for (int h = 0; h < 6; h++) {
+          V8 v;
+          v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + h) * kHalo + r16 + dx) * kLdsRow + 8 * g]);
+          A[h] = v.v;
+        }
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+          const int t = dy * 3 + dx;
+          bf8 B[CT];
+#pragma unroll
+          for (int j = 0; j < CT; j++) {
+            V8 v;
+            v.u = *reinterpret_cast<const uint4*>(&s_w[(t * COB + 16 * j + r16) * kLdsRow + 8 * g]);
+            B[j] = v.v;
+          }
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < CT; j++)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i + dy], B[j], acc[i][j], 0, 0, 0);
+        }
+      }
+    } else {
+      // 1x1: the centre of the halo
       bf8 B[CT];
 #pragma unroll
       for (int j = 0; j < CT; j++) {
         V8 v;
-        v.u = *reinterpret_cast<const uint4*>(&s_w[(t * COB + 16 * j + r16) * kLdsRow + 8 * g]);
+        v.u = *reinterpret_cast<const uint4*>(&s_w[(16 * j + r16) * kLdsRow + 8 * g]);
         B[j] = v.v;
       }
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         V8 v;
-        v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + i + hy) * kHalo + r16 + hx) * kLdsRow + 8 * g]);
+        v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + i + 1) * kHalo + r16 + 1) * kLdsRow + 8 * g]);
 #pragma unroll
         for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v.v, B[j], acc[i][j], 0, 0, 0);
       }
