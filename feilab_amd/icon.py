@@ -95,6 +95,8 @@ def lib():
         "icon_lib_2d_crossv": [ci, ci, ci, ci, ctypes.c_char_p, vp, ctypes.c_char_p, ctypes.c_char_p,
                                ctypes.POINTER(ci), ctypes.POINTER(cf), ci],
         "icon_frc": [vp, vp, ci, ci, vp],
+        "icon_preprocess": [ci, vp, ci, ci, ci, vp, ci, vp, vp],
+        "icon_preprocess_file": [ci, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ci],
         "icon_read_tilts": [ctypes.c_char_p, ctypes.POINTER(cf), ci, ctypes.POINTER(ci)],
         "icon_partition_slices": [ci, ctypes.POINTER(ci), ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)],
         "icon_geometry_tables": [ci, ci, ci, ctypes.POINTER(cf), ctypes.POINTER(PlanInfo)] + [vp] * 10,
@@ -107,6 +109,7 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = ci
+    L.icon_preprocess_error.restype = ctypes.c_char_p
     L.icon_frc_smooth.argtypes = [vp, ci, ci]
     L.icon_frc_smooth.restype = None
     _lib = L
@@ -409,6 +412,28 @@ def frc_smooth(curve, overlap=10):
     c = np.ascontiguousarray(curve, np.float32).copy()
     lib().icon_frc_smooth(c.ctypes.data_as(ctypes.c_void_p), len(c), int(overlap))
     return c
+
+
+def preprocess(stack, tilt_deg=None, thickness=0, device=0):
+    """ICONPreProcess on the device (ICONPreProcess.c:31-86): stack [nz][ny][nx]
+    -> (processed stack, subtracted modes [nz])."""
+    st = np.ascontiguousarray(stack, np.float32)
+    nz, ny, nx = st.shape
+    d = DeviceArray.from_host(st)
+    modes = np.zeros(nz)
+    tl = None if tilt_deg is None else np.ascontiguousarray(tilt_deg, np.float32)
+    rc = lib().icon_preprocess(device, d.ptr, nx, ny, nz, None if tl is None else tl.ctypes.data_as(ctypes.c_void_p),
+                               int(thickness), modes.ctypes.data_as(ctypes.c_void_p), None)
+    if rc:
+        raise IconError(rc, lib().icon_preprocess_error().decode(errors="replace"))
+    return d.numpy(), modes
+
+
+def preprocess_file(in_path, out_path, tilt_path=None, thickness=0, device=0):
+    rc = lib().icon_preprocess_file(device, in_path.encode(), out_path.encode(),
+                                    None if tilt_path is None else tilt_path.encode(), int(thickness))
+    if rc:
+        raise IconError(rc, lib().icon_preprocess_error().decode(errors="replace"))
 
 
 def lib_2d_crossv(mrc_path, angle_path, result_dir, slice_begin, slice_end, pre, icon, post, data_type=1,

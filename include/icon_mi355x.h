@@ -207,6 +207,21 @@ int icon_frc(const float* image1, const float* image2, int nx, int ny, float* fr
 /* frcSmooth (ICONGPU.cu:59-91): in-place running mean over `overlap`
  * entries, the window shifted inwards at the ends. */
 void icon_frc_smooth(float* frc, int size, int overlap);
+/* ICONPreProcess (ICONPreProcess.c:31-86,256-302) on the device: for each
+ * image k of the stack d_stack [nz][ny][nx] (in place), subtract the mode of
+ * a 256-bin histogram of its central region (10 % margins left out; the bin
+ * of the maximum itself is not counted, as in the reference), and, when
+ * tilt_deg != NULL and thickness > 0, scale the image to variance
+ * 0.33 * thickness / cos(-tilt_deg[k] degrees).  modes (host, may be NULL)
+ * receives the subtracted modes.  Synchronous. */
+int icon_preprocess(int device, float* d_stack, int nx, int ny, int nz, const float* tilt_deg, int thickness,
+                    double* modes, void* stream);
+/* The same on MRC files: in_path -> out_path (float, header copied from the
+ * input, statistics recomputed); tilt_path NULL or thickness <= 0: mode
+ * subtraction only. */
+int icon_preprocess_file(int device, const char* in_path, const char* out_path, const char* tilt_path,
+                         int thickness);
+const char* icon_preprocess_error(void);
 /* Last failure of icon_lib_2d2 / icon_lib_2d_crossv on any thread. */
 const char* icon_lib_2d2_error(void);
 

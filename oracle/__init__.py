@@ -64,6 +64,9 @@ def lib():
                                            ctypes.c_int, ctypes.c_float, _f32p, vp, ctypes.c_int]
         L.orc_icon_solve_batch.restype = ctypes.c_int
         L.orc_reproject.argtypes = [vp, _f32p, _f32p]
+        L.orc_preprocess.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p]
+        L.orc_preprocess.restype = None
         _lib = L
     return _lib
 
@@ -250,3 +253,14 @@ def ref_mrc_lib():
     if not os.path.exists(_REF_MRC):
         return None
     return ctypes.CDLL(_REF_MRC)
+
+
+def preprocess(stack, tilt_deg=None, thickness=0):
+    """ICONPreProcess of a stack [nz][ny][nx]: (processed stack, modes[nz])."""
+    st = np.ascontiguousarray(stack, np.float32).copy()
+    nz, ny, nx = st.shape
+    modes = np.zeros(nz)
+    tl = None if tilt_deg is None else np.ascontiguousarray(tilt_deg, np.float32)
+    lib().orc_preprocess(st.reshape(-1), nx, ny, nz, None if tl is None else tl.ctypes.data_as(ctypes.c_void_p),
+                         int(thickness), modes.ctypes.data_as(ctypes.c_void_p))
+    return st, modes

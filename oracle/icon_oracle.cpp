@@ -818,6 +818,58 @@ int orc_icon_solve_batch(void* h, int S, const float* sinos, int pre, int icon, 
   return used;
 }
 
+// ICONPreProcess (sourceCode/ICONPreProcess/ICONPreProcess.c:31-86, 256-302),
+// per section k of a stack [nz][ny][nx]:
+//   minusmode: min/max over the central region (10 % margins excluded, in
+//   double), a 256-bin histogram of that region with bin floor((v-min)/step),
+//   step = (max-min)/256; the reference's index 256 (v == max) lands past its
+//   array and is never read, so it is not counted here; mode = first bin of
+//   the largest count * step + min; v -= mode over the whole section.
+//   normalize (tilts != NULL and thickness > 0): mean = sum (float)(v/size),
+//   var = sum (v-mean)^2/size (double, in index order), v = v/sqrt(var) *
+//   sqrt(0.33 thickness / cos(theta/180*3.1415926)), theta the negated angle.
+// Parity: pinned by closed forms (tests/test_oracle.py); the reference
+// program needs generated code to build.
+void orc_preprocess(float* stack, int nx, int ny, int nz, const float* tilt_deg, int thickness, double* modes) {
+  const int xe = (int)(nx * 0.1), ye = (int)(ny * 0.1);
+  const size_t size = (size_t)nx * ny;
+  for (int k = 0; k < nz; k++) {
+    float* in = stack + (size_t)k * size;
+    double mn = in[(size_t)ye * nx + xe], mx = mn;
+    for (int j = ye; j < ny - ye; j++)
+      for (int i = xe; i < nx - xe; i++) {
+        const double v = in[(size_t)j * nx + i];
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+      }
+    const double step = (mx - mn) / 256.0;
+    double mode = mn;
+    if (step > 0) {
+      std::vector<int> hist(256, 0);
+      for (int j = ye; j < ny - ye; j++)
+        for (int i = xe; i < nx - xe; i++) {
+          const int b = (int)std::floor(((double)in[(size_t)j * nx + i] - mn) / step);
+          if (b >= 0 && b < 256) hist[b]++;
+        }
+      int best = -1000, bi = 0;
+      for (int b = 0; b < 256; b++)
+        if (best < hist[b]) { best = hist[b]; bi = b; }
+      mode = bi * step + mn;
+    }
+    if (modes) modes[k] = mode;
+    for (size_t i = 0; i < size; i++) in[i] = (float)((double)in[i] - mode);
+    if (tilt_deg && thickness > 0) {
+      const double angle = -(double)tilt_deg[k];
+      const double factor = 0.33 * thickness / std::cos(angle / 180.0 * 3.1415926);
+      double sum = 0;
+      for (size_t i = 0; i < size; i++) sum += (double)(in[i] / (float)size);   // float division, as written
+      double var = 0;
+      for (size_t i = 0; i < size; i++) var += ((double)in[i] - sum) * ((double)in[i] - sum) / (double)size;
+      for (size_t i = 0; i < size; i++) in[i] = (float)((double)in[i] / std::sqrt(var) * std::sqrt(factor));
+    }
+  }
+}
+
 void orc_reproject(void* h, const float* m, float* out) {
   auto* a = (AnyPlan*)h;
   if (a->p32) reproject(*a->p32, m, out); else reproject(*a->p64, m, out);

@@ -193,3 +193,11 @@ def test_frc_smooth_window():
     # i=5: [3,7) -> 4.5; i=6: [4,8) clipped to [3,7) -> 4.5; i=7: same
     np.testing.assert_allclose(s, [1.5, 1.5, 1.5, 2.5, 3.5, 4.5, 4.5, 4.5])
     np.testing.assert_allclose(feilab_amd.icon.frc_smooth(np.full(30, 0.25, np.float32), 10), 0.25)
+
+
+def test_preprocess_cli_help_and_missing_flags():
+    exe = os.path.join(ROOT, "feilab_amd", "icon-preprocess")
+    r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 0 and "-thickness (-th)" in r.stdout
+    r = subprocess.run([exe, "-i", "x.mrc"], capture_output=True, text=True, timeout=30)
+    assert "parameter error!" in r.stdout

@@ -230,3 +230,54 @@ def test_unet_oracle_gradient_matches_finite_differences():
     _, gb = unet_ref.gradients(p, x, t, bf16=True)
     assert np.linalg.norm(gb - g) / np.linalg.norm(g) < 0.1
     assert float(gb @ g / (np.linalg.norm(gb) * np.linalg.norm(g))) > 0.99
+
+
+def _central(nx, ny):
+    return slice(int(ny * 0.1), ny - int(ny * 0.1)), slice(int(nx * 0.1), nx - int(nx * 0.1))
+
+
+def test_preprocess_mode_is_lower_edge_of_the_fullest_central_bin(oracle_mod):
+    """ICONPreProcess mode: 256 bins over [min, max] of the central region
+    (margins excluded); the mode is the lower edge of the fullest bin and is
+    subtracted everywhere."""
+    rng = np.random.default_rng(3)
+    ny, nx = 50, 60
+    img = rng.uniform(0.0, 1.0, (ny, nx)).astype(np.float32)
+    ys, xs = _central(nx, ny)
+    cen = img[ys, xs]
+    cen[rng.uniform(size=cen.shape) < 0.2] = np.float32(0.3017)   # a dominant value
+    img[ys, xs] = cen
+    img[0, :] = 50.0      # margin outliers: ignored
+    img[:, -1] = -50.0
+    out, modes = oracle_mod.preprocess(img[None])
+    mn, mx = float(cen.min()), float(cen.max())
+    step = (mx - mn) / 256
+    assert 0 <= 0.3017 - modes[0] < step + 1e-12
+    assert abs((modes[0] - mn) / step - round((modes[0] - mn) / step)) < 1e-9   # a bin edge
+    np.testing.assert_array_equal(out[0], (img.astype(np.float64) - modes[0]).astype(np.float32))
+
+
+def test_preprocess_ignores_the_maximum_bin_and_constant_images(oracle_mod):
+    rng = np.random.default_rng(4)
+    ny, nx = 40, 40
+    img = rng.uniform(0.0, 0.5, (ny, nx)).astype(np.float32)
+    ys, xs = _central(nx, ny)
+    cen = img[ys, xs]
+    cen[rng.uniform(size=cen.shape) < 0.6] = 1.0      # the maximum, most frequent value
+    cen[0, 0] = 0.2                                   # second most frequent cluster
+    cen[rng.uniform(size=cen.shape) < 0.1] = 0.2
+    img[ys, xs] = cen
+    _, modes = oracle_mod.preprocess(img[None])
+    step = (1.0 - float(cen.min())) / 256
+    assert abs(modes[0] - 0.2) < step          # not the maximum
+    out, modes = oracle_mod.preprocess(np.full((2, 20, 30), 7.5, np.float32))
+    assert np.all(modes == 7.5) and np.all(out == 0)
+
+
+def test_preprocess_normalises_variance_to_thickness_over_cosine(oracle_mod):
+    rng = np.random.default_rng(5)
+    st = (rng.standard_normal((3, 48, 64)) * [[[2.0]], [[0.5]], [[1.0]]] + 3).astype(np.float32)
+    tilt = np.array([30.0, 0.0, -45.0], np.float32)
+    out, _ = oracle_mod.preprocess(st, tilt, 120)
+    want = 0.33 * 120 / np.cos(-tilt / 180.0 * 3.1415926)
+    np.testing.assert_allclose(out.reshape(3, -1).astype(np.float64).var(axis=1), want, rtol=1e-5)
