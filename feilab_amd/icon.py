@@ -97,6 +97,10 @@ def lib():
         "icon_frc": [vp, vp, ci, ci, vp],
         "icon_preprocess": [ci, vp, ci, ci, ci, vp, ci, vp, vp],
         "icon_preprocess_file": [ci, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ci],
+        "icon_mask3_weights": [ci, vp, ci, cf, cf, ci, ci, cf, vp],
+        "icon_mask3_filter": [ci, vp, ci, ci, vp, vp],
+        "icon_mask3": [ci, ctypes.c_char_p, ci, ci, ci, ctypes.c_char_p, ctypes.c_char_p, ci, ctypes.c_char_p,
+                       ctypes.c_char_p, ci, ci, ci],
         "icon_read_tilts": [ctypes.c_char_p, ctypes.POINTER(cf), ci, ctypes.POINTER(ci)],
         "icon_partition_slices": [ci, ctypes.POINTER(ci), ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)],
         "icon_geometry_tables": [ci, ci, ci, ctypes.POINTER(cf), ctypes.POINTER(PlanInfo)] + [vp] * 10,
@@ -110,6 +114,7 @@ def lib():
         fn.argtypes = args
         fn.restype = ci
     L.icon_preprocess_error.restype = ctypes.c_char_p
+    L.icon_mask3_error.restype = ctypes.c_char_p
     L.icon_frc_smooth.argtypes = [vp, ci, ci]
     L.icon_frc_smooth.restype = None
     _lib = L
@@ -434,6 +439,28 @@ def preprocess_file(in_path, out_path, tilt_path=None, thickness=0, device=0):
                                     None if tilt_path is None else tilt_path.encode(), int(thickness))
     if rc:
         raise IconError(rc, lib().icon_preprocess_error().decode(errors="replace"))
+
+
+def mask3_weights(N, tilt_deg, radius05_rate, radius03_rate, radius_given, gauss_width, mean_step):
+    """ICONMask3's half-spectrum multiplier [N][N/2+1] (host computation)."""
+    t = np.ascontiguousarray(tilt_deg, np.float32)
+    out = np.zeros((N, N // 2 + 1), np.float32)
+    rc = lib().icon_mask3_weights(N, t.ctypes.data_as(ctypes.c_void_p), len(t), radius05_rate, radius03_rate,
+                                  int(radius_given), int(gauss_width), mean_step, out.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise IconError(rc, lib().icon_mask3_error().decode(errors="replace"))
+    return out
+
+
+def mask3_filter(slices, half_mask, device=0):
+    """ICONMask3's filtering of [S][N][N] real slices on the device."""
+    sl = np.ascontiguousarray(slices, np.float32)
+    d = DeviceArray.from_host(sl)
+    hm = np.ascontiguousarray(half_mask, np.float32)
+    rc = lib().icon_mask3_filter(device, d.ptr, sl.shape[-1], sl.shape[0], hm.ctypes.data_as(ctypes.c_void_p), None)
+    if rc:
+        raise IconError(rc, lib().icon_mask3_error().decode(errors="replace"))
+    return d.numpy()
 
 
 def lib_2d_crossv(mrc_path, angle_path, result_dir, slice_begin, slice_end, pre, icon, post, data_type=1,

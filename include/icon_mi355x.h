@@ -222,6 +222,29 @@ int icon_preprocess(int device, float* d_stack, int nx, int ny, int nz, const fl
 int icon_preprocess_file(int device, const char* in_path, const char* out_path, const char* tilt_path,
                          int thickness);
 const char* icon_preprocess_error(void);
+/* ICONMask3 (ICONCrossVMaskIn2D_weightOnlyOnSampled.cu:369-598,602-1236).
+ * icon_mask3_weights: the half-spectrum multiplier [N][N/2+1] (row ky,
+ *   column kx <= N/2) = realMask * weighting of genMask: the angular sampling
+ *   mask of the tilts (degrees as in the tilt file), 1 inside the FRC-0.5
+ *   radius radius05_rate * N/2, a Gaussian edge out to the FRC-0.3 radius
+ *   (or gauss_width >= 0 pixels; radius_given selects the '-radius' rules),
+ *   times |k|/r_full on sampled frequencies beyond r_full.  Host only.
+ * icon_mask3_filter: in place on `count` real N x N slices on the device
+ *   (N 8..1024, or a power of two up to 2048): the reference's R2C ->
+ *   multiply -> C2R / N^2.  Synchronous.
+ * icon_mask3: the program on <input_dir>/midYYYYY.mrc, slices slice_begin ..
+ *   slice_end (inclusive), cropped to `thickness` rows around the centre
+ *   (+ zshift), written to out_path.  radius > 0 gives the mask radius;
+ *   otherwise it comes from the crossV / fullRec FRC files (falling back to
+ *   the tilt range's radius when they are not valid); no_mask != 0 only
+ *   crops and stacks (simpleCatch). */
+int icon_mask3_weights(int N, const float* tilt_deg, int ang_num, float radius05_rate, float radius03_rate,
+                       int radius_given, int gauss_width, float mean_ang_step, float* half_mask);
+int icon_mask3_filter(int device, float* d_slices, int N, int count, const float* half_mask, void* stream);
+int icon_mask3(int device, const char* input_dir, int slice_begin, int slice_end, int thickness,
+               const char* tilt_path, const char* out_path, int radius, const char* crossv_frc,
+               const char* fullrec_frc, int zshift, int gauss_width, int no_mask);
+const char* icon_mask3_error(void);
 /* Last failure of icon_lib_2d2 / icon_lib_2d_crossv on any thread. */
 const char* icon_lib_2d2_error(void);
 

@@ -201,3 +201,28 @@ def test_preprocess_cli_help_and_missing_flags():
     assert r.returncode == 0 and "-thickness (-th)" in r.stdout
     r = subprocess.run([exe, "-i", "x.mrc"], capture_output=True, text=True, timeout=30)
     assert "parameter error!" in r.stdout
+
+
+def test_mask3_host_weights_match_oracle(icon):
+    """The library's ICONMask3 mask construction (host C++) against the
+    restatement, for the FRC-radius and the given-radius rules."""
+    from oracle import mask3_ref
+    tilts = np.array([-58.2, -55.1, -51.9, -49.0] + list(np.arange(-45, 46, 3.1)) + [48.7, 52.2, 57.9], np.float32)
+    big, mean = mask3_ref.tilt_stats(tilts)
+    for N, args in ((128, (0.31, 0.52, 0, -1)), (256, (22 / 128, 0.0, 1, -1)), (128, (0.2, 0.4, 0, 7)),
+                    (100, (0.25, 0.45, 0, -1))):
+        got = icon.icon.mask3_weights(N, tilts, args[0], args[1], args[2], args[3], mean)
+        want = mask3_ref.weights(N, tilts, args[0], args[1], bool(args[2]), args[3], mean)
+        np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-7)
+
+
+def test_mask3_cli_help_and_missing_flags():
+    exe = os.path.join(ROOT, "feilab_amd", "icon-mask3")
+    r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1 and "-crossVfrc (-cf)" in r.stdout
+    r = subprocess.run([exe, "-i", "d", "-s", "0,3"], capture_output=True, text=True, timeout=30)
+    assert r.returncode != 0 and "are missed!" in r.stdout
+    # a mask needs '-radius' or both FRC files
+    r = subprocess.run([exe, "-i", "d", "-s", "0,3", "-th", "8", "-t", "a.tlt", "-o", "o.mrc"],
+                       capture_output=True, text=True, timeout=30)
+    assert r.returncode != 0 and "'-radius' or ('-crossVfrc' & 'fullRecfrc') are missed!" in r.stdout

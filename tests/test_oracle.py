@@ -4,6 +4,8 @@ invariants and special cases that reduce to textbook routines.  Each check is
 chosen so a plausible slip (a dropped factor, a wrong sign or index, a
 transposed operand) fails it.  CPU only.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -281,3 +283,63 @@ def test_preprocess_normalises_variance_to_thickness_over_cosine(oracle_mod):
     out, _ = oracle_mod.preprocess(st, tilt, 120)
     want = 0.33 * 120 / np.cos(-tilt / 180.0 * 3.1415926)
     np.testing.assert_allclose(out.reshape(3, -1).astype(np.float64).var(axis=1), want, rtol=1e-5)
+
+
+def test_mask3_sampling_mask_is_the_tilt_lines():
+    """One tilt: the mask is a single line through the centre, perpendicular
+    to the rotation by -theta-90 of the central column (theta = 0: the
+    central row), with unit values on the line; many tilts cover a wedge and
+    stay clipped at 1."""
+    from oracle import mask3_ref
+    N = 64
+    m = mask3_ref.sampling_mask(N, [0.0])
+    assert np.allclose(m[N // 2, 1:-1], 1.0, atol=1e-5)
+    assert m.sum() == pytest.approx(m[N // 2].sum() + m[N // 2 + 1].sum() + m[N // 2 - 1].sum(), rel=1e-6)
+    m90 = mask3_ref.sampling_mask(N, [90.0])
+    assert np.allclose(m90[1:-1, N // 2], 1.0, atol=1e-5)
+    wedge = mask3_ref.sampling_mask(N, np.arange(-60, 61, 3))
+    assert wedge.max() <= 1.0 and wedge[N // 2, N // 2 + 10] == pytest.approx(1.0)
+    assert wedge[N // 2 + 20, N // 2] == 0.0            # inside the missing wedge
+
+
+def test_mask3_weights_closed_forms():
+    """realMask = 1 inside the FRC-0.5 radius, the Gaussian edge is 1 at that
+    radius and reaches the sampling mask at radius + width; weighting = |k| /
+    r_full on sampled frequencies beyond r_full."""
+    from oracle import mask3_ref
+    N = 128
+    tilts = np.arange(-60, 61, 3, dtype=np.float32)
+    _, mean = mask3_ref.tilt_stats(tilts)
+    r05, r03 = 20 / 64, 40 / 64
+    w = mask3_ref.weights(N, tilts, r05, r03, False, -1, mean)
+    r_full = math.ceil(2.0 / (float(mean) / 360.0 * 2 * 3.1415926))
+    om = mask3_ref.sampling_mask(N, tilts)
+    # (ky = 0, kx = r): centred index (N/2, N/2 + r), on a sampled line
+    for r in (5, 15, 19):
+        assert w[0, r] == pytest.approx(max(1.0, r / r_full) if om[N // 2, N // 2 + r] > 1e-5 else 1.0)
+    for r in (45, 60):   # beyond the edge: mask x weighting
+        expect = om[N // 2, N // 2 + r] * (r / r_full if (r > r_full and om[N // 2, N // 2 + r] > 1e-5) else 1.0)
+        assert w[0, r] == pytest.approx(expect, rel=1e-6)
+    # inside the missing wedge (kx = 0, ky = 30): realMask from the edge only
+    assert w[30, 0] < 1.0
+
+
+def test_mask3_radii_from_frc():
+    from oracle import mask3_ref
+    N = 64
+    x = np.arange(N // 2) / (N // 2)
+    cv = np.clip(1.2 - 1.5 * x, 0, 1)
+    fr = np.ones(N // 2)
+    r05, r03, ok = mask3_ref.radii_from_frc(cv, fr, N)
+    assert ok and r05 == int(np.nonzero(cv >= 0.5)[0][-1]) and r03 == int(np.nonzero(cv >= 0.3)[0][-1])
+    assert not mask3_ref.radii_from_frc(cv[:10], fr[:10], N)[2]        # too short
+    only0 = np.zeros(N // 2)
+    only0[0] = 0.4
+    assert mask3_ref.radii_from_frc(only0, fr, N) == (-1, 0, False)   # FRC-0.3 radius 0
+    crossed = np.where(x < 0.5, 0.4, 0.0)
+    crossed[:3] = 0.6
+    crossed[20:] = 0.0
+    assert mask3_ref.radii_from_frc(crossed, fr, N)[2]
+    # a curve below 0.3 everywhere gives -1 for both radii, which the reference's check
+    # (radius05 > radius03 or radius03 == 0) lets through
+    assert mask3_ref.radii_from_frc(np.zeros(N // 2), fr, N) == (-1, -1, True)
