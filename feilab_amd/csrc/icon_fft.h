@@ -110,15 +110,28 @@ __host__ __device__ constexpr int last_radix() {
   return r;
 }
 
-// Loads a pass's inputs from LDS (element stride ES float2).
-template <int N, int RP, int ES>
+// LDS layout of a line: element i at float2 offset i*ES, plus PN padding
+// float2 after every 2^PSH elements (PN = 0: no padding).  The padding moves
+// elements that a pass's stores or loads place 2^PSH apart onto different
+// banks; its offset (i >> PSH) * PN splits into a per-thread base and a
+// compile-time constant at every access of the passes below.
+template <int ES, int PSH = 0, int PN = 0>
+struct Lay {
+  static constexpr int kES = ES;
+  __host__ __device__ static constexpr int at(int i) { return i * ES + (PN ? (i >> PSH) * PN : 0); }
+  // float2 needed for a line of n elements
+  __host__ __device__ static constexpr int size(int n) { return at(n - 1) + 1; }
+};
+
+// Loads a pass's inputs from LDS (layout L).
+template <int N, int RP, typename L>
 __device__ __forceinline__ void pass_load(float2 (&v)[16], const float2* lds, int t) {
   constexpr int T = N / 16;
   constexpr int B = 16 / RP;
 #pragma unroll
   for (int q = 0; q < B; q++)
 #pragma unroll
-    for (int r = 0; r < RP; r++) v[q * RP + r] = lds[(t + q * T + r * (N / RP)) * ES];
+    for (int r = 0; r < RP; r++) v[q * RP + r] = lds[L::at(t + q * T + r * (N / RP))];
 }
 
 // Twiddles of the passes after the first (at most two for N <= 4096), per
@@ -171,8 +184,8 @@ __device__ __forceinline__ void pass_compute(float2 (&v)[16], const FftTwiddles&
   }
 }
 
-// Stores a pass's outputs to LDS in Stockham order.
-template <int N, int RP, int NS, int ES>
+// Stores a pass's outputs to LDS in Stockham order (layout L).
+template <int N, int RP, int NS, typename L>
 __device__ __forceinline__ void pass_store(const float2 (&v)[16], float2* lds, int t) {
   constexpr int T = N / 16;
   constexpr int B = 16 / RP;
@@ -182,7 +195,7 @@ __device__ __forceinline__ void pass_store(const float2 (&v)[16], float2* lds, i
     const int k = j & (NS - 1);
     const int base = (j - k) * RP + k;
 #pragma unroll
-    for (int r = 0; r < RP; r++) lds[(base + r * NS) * ES] = v[q * RP + r];
+    for (int r = 0; r < RP; r++) lds[L::at(base + r * NS)] = v[q * RP + r];
   }
 }
 
@@ -201,34 +214,33 @@ __device__ __forceinline__ void line_sync() {
 
 // Runs the passes from sub-transform size NS on; v holds that pass's inputs.
 // Every thread of the block (WAVE = 0) or of the wave (WAVE = 1) must call it.
-template <int N, int ES, int DIR, int NS, int WAVE = 0, int P = 0>
+template <int N, typename L, int DIR, int NS, int WAVE = 0, int P = 0>
 __device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {
   constexpr int RP = pass_radix<N, NS>();
   pass_compute<N, RP, NS, DIR, P>(v, T);
   if constexpr (NS * RP < N) {
-    pass_store<N, RP, NS, ES>(v, lds, t);
+    pass_store<N, RP, NS, L>(v, lds, t);
     line_sync<WAVE>();
     constexpr int RN = pass_radix<N, NS * RP>();
-    pass_load<N, RN, ES>(v, lds, t);
+    pass_load<N, RN, L>(v, lds, t);
     line_sync<WAVE>();
-    fft_from<N, ES, DIR, NS * RP, WAVE, (NS > 1 ? P + 1 : 0)>(v, lds, t, T);
+    fft_from<N, L, DIR, NS * RP, WAVE, (NS > 1 ? P + 1 : 0)>(v, lds, t, T);
   }
 }
 
-// Registers in (input order) -> registers out (output order).  lds: N
-// float2 of scratch for this line at element stride ES; T from
-// load_twiddles<N, DIR>.
-template <int N, int ES, int DIR>
+// Registers in (input order) -> registers out (output order).  lds: scratch
+// for this line in layout L (L::size(N) float2); T from load_twiddles<N, DIR>.
+template <int N, typename L, int DIR>
 __device__ __forceinline__ void fft_regs(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {
-  fft_from<N, ES, DIR, 1>(v, lds, t, T);
+  fft_from<N, L, DIR, 1>(v, lds, t, T);
 }
 
 // The same, loading the twiddles itself.
-template <int N, int ES, int DIR>
+template <int N, typename L, int DIR>
 __device__ __forceinline__ void fft_regs(float2 (&v)[16], float2* lds, int t, const float2* __restrict__ tw) {
   FftTwiddles T;
   load_twiddles<N, DIR>(T, t, tw);
-  fft_from<N, ES, DIR, 1>(v, lds, t, T);
+  fft_from<N, L, DIR, 1>(v, lds, t, T);
 }
 
 // Element index held in output slot i of thread t.
@@ -245,36 +257,21 @@ __device__ __forceinline__ int in_index(int t, int r) {
   return t + r * (N / 16);
 }
 
-// LDS line in natural order -> transformed line in natural order (in place).
-// Callers synchronise before (data present) and after (data consumed).
-template <int N, int ES, int DIR>
+// LDS line in natural order -> transformed line in natural order (in place,
+// layout L).  Callers synchronise before (data present) and after (data
+// consumed).
+template <int N, typename L, int DIR>
 __device__ __forceinline__ void fft_lds(float2* lds, int t, const float2* __restrict__ tw) {
   FftTwiddles T;
   load_twiddles<N, DIR>(T, t, tw);
   float2 v[16];
-  pass_load<N, 16, ES>(v, lds, t);
+  pass_load<N, 16, L>(v, lds, t);
   __syncthreads();
-  fft_regs<N, ES, DIR>(v, lds, t, T);
+  fft_regs<N, L, DIR>(v, lds, t, T);
   // the last pass_load is followed by a barrier inside fft_from, so the
   // line can be overwritten now
 #pragma unroll
-  for (int i = 0; i < 16; i++) lds[out_index<N>(t, i) * ES] = v[i];
-}
-
-// The same with one wavefront per line (N / 16 == 64 threads, t = lane):
-// the exchanges need only wavefront barriers.  The caller synchronises the
-// block before (the line is complete in LDS) and after (it is read by others).
-template <int N, int ES, int DIR>
-__device__ __forceinline__ void fft_lds_wave(float2* lds, int t, const float2* __restrict__ tw) {
-  static_assert(N / 16 == 64, "one wavefront per line");
-  FftTwiddles T;
-  load_twiddles<N, DIR>(T, t, tw);
-  float2 v[16];
-  pass_load<N, 16, ES>(v, lds, t);
-  line_sync<1>();
-  fft_from<N, ES, DIR, 1, 1>(v, lds, t, T);
-#pragma unroll
-  for (int i = 0; i < 16; i++) lds[out_index<N>(t, i) * ES] = v[i];
+  for (int i = 0; i < 16; i++) lds[L::at(out_index<N>(t, i))] = v[i];
 }
 
 }  // namespace icon

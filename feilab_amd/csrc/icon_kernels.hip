@@ -38,12 +38,18 @@ namespace icon {
 
 // ------------------------------------------------------------------ row pass
 // One grid row of length NR per team of NR/16 threads; RPB rows per block.
+// LDS rows carry one float2 of padding after every 32 elements: the first
+// pass stores 16 consecutive elements per thread, which without it would put
+// every second thread of a wavefront on the same banks.
+using RowLay = Lay<1, 5, 1>;
+template <int NR>
+constexpr int kRowLds = RowLay::size(NR);
 
 
 template <int NR, int RPB>
 __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
   constexpr int T = NR / 16;
-  __shared__ float2 lds[RPB * NR];
+  __shared__ float2 lds[RPB * kRowLds<NR>];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int row = blockIdx.x * RPB + team;
@@ -84,7 +90,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
     }
     v[r] = gv;
   }
-  fft_regs<NR, 1, -1>(v, lds + team * NR, t, Tw);
+  fft_regs<NR, RowLay, -1>(v, lds + team * kRowLds<NR>, t, Tw);
   if (valid) {
     float2* o = a.out + (size_t)row * NR;
 #pragma unroll
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
 template <int NR, int RPB>
 __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
   constexpr int T = NR / 16;
-  __shared__ float2 lds[RPB * NR];
+  __shared__ float2 lds[RPB * kRowLds<NR>];
   __shared__ double red[RPB * T];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -110,7 +116,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
   const float2* src = a.in + (size_t)(valid ? row : 0) * NR;
 #pragma unroll
   for (int r = 0; r < 16; r++) v[r] = valid ? src[in_index<NR>(t, r)] : make_float2(0.f, 0.f);
-  fft_regs<NR, 1, 1>(v, lds + team * NR, t, Tw);
+  fft_regs<NR, RowLay, 1>(v, lds + team * kRowLds<NR>, t, Tw);
   const int h1 = a.N1 >> 1;
   double acc = 0.0;
   if (valid) {
@@ -159,11 +165,25 @@ __host__ __device__ constexpr int col_stride() {
   return NC * (C + 2) * 8 <= 81920 ? C + 2 : C + 1;
 }
 
+// Layout of the tile during the x FFT's exchanges: row stride C + 1 and 8
+// float2 of padding after every 16 rows.  A wavefront holds 64/C threads
+// per column whose first-pass outputs start 16 rows apart; the padding puts
+// them on different banks (the unpadded tile would conflict 64/C ways).
+template <int C>
+using ColFftLay = Lay<C + 1, 4, 8>;
+
+// float2 of LDS per column block: the tile, or the FFT layout if larger
 template <int NC, int C>
-__global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
+__host__ __device__ constexpr int col_lds() {
+  return NC * col_stride<NC, C>() > ColFftLay<C>::at(NC - 1) + C ? NC * col_stride<NC, C>()
+                                                                 : ColFftLay<C>::at(NC - 1) + C;
+}
+
+template <int NC, int C>
+__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   constexpr int NT = C * (NC / 16);
   constexpr int LS = col_stride<NC, C>();
-  extern __shared__ __align__(16) float2 tile[];   // NC * LS
+  extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
   // tile-major order: blocks in flight together work on the same column
   // tile of different slices and share its bin records in L2
   const int blk = blockIdx.x / a.S;
@@ -186,7 +206,7 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
       const int X = (x + h0) & (NC - 1);
       v[r] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
     }
-    fft_regs<NC, LS, -1>(v, tile + c, t, T);
+    fft_regs<NC, ColFftLay<C>, -1>(v, tile + c, t, T);
     // the last exchange ended with a barrier: the column can be overwritten
 #pragma unroll
     for (int i = 0; i < 16; i++) tile[out_index<NC>(t, i) * LS + c] = v[i];
@@ -213,8 +233,10 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
       const float4 v = pyv[c4];
       py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
     }
+    // separable window: each grid row is first reduced over its C y-taps,
+    // then weighted by its x-tap
     float ax = 0.f, ay = 0.f;
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < kTapsK; q++) {
       const float2* row = tile + ((en.z + q) & (NC - 1)) * LS;
       float2 gv[C];
@@ -229,129 +251,121 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_fwd(ColArgs a) {
 #pragma unroll
         for (int c = 0; c < C; c++) gv[c] = row[c];
       }
+      float tx = 0.f, ty = 0.f;
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        const float phi = px[q] * py[c];
-        ax = __fmaf_rn(gv[c].x, phi, ax);
-        ay = __fmaf_rn(gv[c].y, phi, ay);
+        tx = __fmaf_rn(gv[c].x, py[c], tx);
+        ty = __fmaf_rn(gv[c].y, py[c], ty);
       }
+      ax = __fmaf_rn(tx, px[q], ax);
+      ay = __fmaf_rn(ty, px[q], ay);
     }
     a.part[((size_t)s * a.M + en.x) * kMaxSlotK + ((en.y >> 16) & 0xff)] = make_float2(ax, ay);
   }
 }
 
-// Wavefront-scope barrier for LDS data exchanged between the lanes of one wave.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int NC, int C>
-__global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
+__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   constexpr int NT = C * (NC / 16);
-  constexpr int LS = col_stride<NC, C>();
-  constexpr int NW = NT / 64;                    // waves per block
-  constexpr int RPT = NC >= NT ? NC / NT : 1;    // grid rows per thread
-  // staged record, 16-byte aligned fields: x-taps [0,14) + 2 pad, y-taps
-  // [16, 16+C), then (f.x, f.y, first x-tap, pad)
+  constexpr int RL = NC / NT;                    // adjacent grid rows per thread
+  static_assert(RL >= 1 && RL <= 2 && NT * RL == NC, "one or two grid rows per thread");
+  // staged record, 16-byte aligned fields: the 14 x-taps at [RL-1, RL+13)
+  // with zeros around them in [0, 16), the y-taps at [16, 16+C), then
+  // (f.x, f.y, first x-tap, pad)
   constexpr int REC = 16 + C + 4;
-  constexpr int REGION = NC * LS * 2 / NW;       // floats of LDS staging per wave
-  constexpr int CH = 64 * REC <= REGION ? 64 : 32;   // records staged per chunk
-  static_assert(CH * REC <= REGION && REGION % 4 == 0, "staging chunk does not fit");
-  extern __shared__ __align__(16) float2 tile[];   // NC * LS
+  constexpr int CAP = col_lds<NC, C>() * 2 / REC;  // records staged at once in the tile's LDS
+  extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
   // tile-major order: blocks in flight together work on the same column
   // tile of different slices and share its bin records in L2
   const int blk = blockIdx.x / a.S;
   const int s = blockIdx.x % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  // Spread as a gather.  Lane x of a wave owns grid row x of the tile (C
-  // cells in registers).  The entries whose 14-row x-window can cover the
-  // wave's 64 rows [xa, xa+64) are the contiguous runs of the tile's x-sorted
-  // list with first x-tap in [xa-13, xa+63] (mod n0); the wave stages them,
-  // 64 at a time, in its share of the (not yet used) tile LDS, and each lane
-  // adds the staged entries that cover its row.  No atomics; every cell sums
-  // its entries in list order.
+  // Spread as a gather.  Thread tid owns the RL adjacent grid rows
+  // [x0, x0+RL) of the tile (C cells each, in registers).  The entries that
+  // cover them are those with first x-tap z in [x0-13, x0+RL-1] (mod n0): one
+  // or two index ranges of the tile's z-sorted list.  The block stages the
+  // tile's records in LDS (CAP at a time) and every thread adds its whole
+  // ranges, two entries per step, each record's taps shared by its RL rows.
+  // No atomics; every cell sums its entries in list order.
   const int* xo = a.xoff + (size_t)blk * (NC + 1);
   const float2* fw = a.fw + (size_t)s * a.E;
-  float* stage = reinterpret_cast<float*>(tile) + wave * REGION;
-  float2 acc[RPT][C];
+  float* stage = reinterpret_cast<float*>(tile);
+  const int x0 = tid * RL;
+  int lo[2], hi[2];
+  if (x0 >= kTapsK - 1) {
+    lo[0] = hi[0] = 0;
+    lo[1] = xo[x0 - (kTapsK - 1)]; hi[1] = xo[x0 + RL];
+  } else {
+    lo[0] = xo[NC - (kTapsK - 1) + x0]; hi[0] = xo[NC];
+    lo[1] = xo[0]; hi[1] = xo[x0 + RL];
+  }
+  float2 acc[RL][C];
 #pragma unroll
-  for (int r = 0; r < RPT; r++) {
+  for (int r = 0; r < RL; r++) {
 #pragma unroll
     for (int c = 0; c < C; c++) acc[r][c] = make_float2(0.f, 0.f);
   }
+  // adds entry record `rec` to the thread's rows
+  auto add = [&](const float* rec) {
+    const float4 tail = reinterpret_cast<const float4*>(rec)[4 + C / 4];
+    // x-tap slot of row x0: (x0 - z) + RL - 1, in [0, 15] by the range
+    const int q = (x0 - __float_as_int(tail.z) + RL - 1) & (NC - 1);
+    float px[RL];
 #pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int xa = r * NT + wave * 64;
-    if (xa >= NC) break;
-    const int x = xa + lane;
-    int lo1, hi1, lo2 = 0, hi2 = 0;
-    if (xa >= kTapsK - 1) {
-      lo1 = xo[xa - (kTapsK - 1)]; hi1 = xo[xa + 64];
-    } else {
-      lo1 = xo[NC - (kTapsK - 1) + xa]; hi1 = xo[NC]; lo2 = xo[0]; hi2 = xo[xa + 64];
+    for (int r = 0; r < RL; r++) px[r] = rec[q + r];
+    float py[C];
+#pragma unroll
+    for (int c4 = 0; c4 < C / 4; c4++) {
+      const float4 v = reinterpret_cast<const float4*>(rec)[4 + c4];
+      py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
     }
-    // this lane's own candidates: the entries with first x-tap in [x-13, x]
-    // (mod NC), as an index range in each of the wave's runs; every one of
-    // them covers row x
-    int llo[2], lhi[2];
-    if (x >= kTapsK - 1) {
-      llo[0] = lhi[0] = 0;
-      llo[1] = xo[x - (kTapsK - 1)]; lhi[1] = xo[x + 1];
-    } else {
-      llo[0] = xo[NC - (kTapsK - 1) + x]; lhi[0] = xo[NC];
-      llo[1] = xo[0]; lhi[1] = xo[x + 1];
-    }
-    for (int run = 0; run < 2; run++) {
-      const int lo = run ? lo2 : lo1, hi = run ? hi2 : hi1;
-      // the wrap run of the first wave is run 0; otherwise the single run is run 1
-      const int lr = (xa >= kTapsK - 1) ? 1 : run;
-      for (int base = lo; base < hi; base += CH) {
-        const int cnt = min(CH, hi - base);
-        wave_sync();
-        if (lane < cnt) {
-          const int e = base + lane;
-          float4* rec = reinterpret_cast<float4*>(stage + lane * REC);
-          const float2* pxv = reinterpret_cast<const float2*>(a.ent_px + (size_t)e * kTapsK);
-          float2 p[kTapsK / 2];
 #pragma unroll
-          for (int q = 0; q < kTapsK / 2; q++) p[q] = pxv[q];
+    for (int r = 0; r < RL; r++) {
+      // the value weighted by this row's x-tap, then by each y-tap
+      const float fx = tail.x * px[r], fy = tail.y * px[r];
 #pragma unroll
-          for (int q = 0; q < 3; q++) rec[q] = make_float4(p[2 * q].x, p[2 * q].y, p[2 * q + 1].x, p[2 * q + 1].y);
-          rec[3] = make_float4(p[6].x, p[6].y, 0.f, 0.f);
-          const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
-#pragma unroll
-          for (int c4 = 0; c4 < C / 4; c4++) rec[4 + c4] = pyv[c4];
-          const float2 f = fw[e];
-          rec[4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
-        }
-        wave_sync();
-        // this lane's part of the chunk, in list order
-        const int k0 = max(llo[lr], base) - base;
-        const int k1 = min(lhi[lr], base + cnt) - base;
-        for (int k = k0; k < k1; k++) {
-          const float* rec = stage + k * REC;
-          const float4 tail = reinterpret_cast<const float4*>(rec)[4 + C / 4];
-          const int tx = (x - __float_as_int(tail.z)) & (NC - 1);   // < 14 by the range
-          const float px = rec[tx];
-          float py[C];
-#pragma unroll
-          for (int c4 = 0; c4 < C / 4; c4++) {
-            const float4 v = reinterpret_cast<const float4*>(rec)[4 + c4];
-            py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
-          }
-#pragma unroll
-          for (int c = 0; c < C; c++) {
-            const float phi = px * py[c];
-            acc[r][c].x = __fmaf_rn(tail.x, phi, acc[r][c].x);
-            acc[r][c].y = __fmaf_rn(tail.y, phi, acc[r][c].y);
-          }
-        }
+      for (int c = 0; c < C; c++) {
+        acc[r][c].x = __fmaf_rn(fx, py[c], acc[r][c].x);
+        acc[r][c].y = __fmaf_rn(fy, py[c], acc[r][c].y);
       }
+    }
+  };
+  const int e0 = a.ent_off[blk], e1 = a.ent_off[blk + 1];
+  for (int cb = e0; cb < e1; cb += CAP) {
+    const int cn = min(CAP, e1 - cb);
+    if (cb > e0) __syncthreads();   // the previous chunk has been consumed
+    for (int i = tid; i < cn; i += NT) {
+      const int e = cb + i;
+      float4* rec = reinterpret_cast<float4*>(stage + i * REC);
+      const float2* pxv = reinterpret_cast<const float2*>(a.ent_px + (size_t)e * kTapsK);
+      float t[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) t[q] = 0.f;
+#pragma unroll
+      for (int q = 0; q < kTapsK / 2; q++) {
+        const float2 v = pxv[q];
+        t[RL - 1 + 2 * q] = v.x;
+        t[RL + 2 * q] = v.y;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) rec[q] = make_float4(t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]);
+      const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
+#pragma unroll
+      for (int c4 = 0; c4 < C / 4; c4++) rec[4 + c4] = pyv[c4];
+      const float2 f = fw[e];
+      rec[4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int run = 0; run < 2; run++) {
+      const int k1 = min(hi[run], cb + cn) - cb;
+      int k = max(lo[run], cb) - cb;
+      for (; k + 1 < k1; k += 2) {
+        add(stage + k * REC);
+        add(stage + (k + 1) * REC);
+      }
+      if (k < k1) add(stage + k * REC);
     }
   }
   // twiddles of the inverse x FFT, in flight while the tile is written
@@ -359,12 +373,9 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
   load_twiddles<NC, 1>(Tw, tid / C, a.tw);
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int x = r * NT + wave * 64 + lane;
-    if (x < NC) {
+  for (int r = 0; r < RL; r++) {
 #pragma unroll
-      for (int c = 0; c < C; c++) tile[x * LS + c] = acc[r][c];
-    }
+    for (int c = 0; c < C; c++) tile[ColFftLay<C>::at(x0 + r) + c] = acc[r][c];
   }
   __syncthreads();
   // inverse x FFT of column c = tid % C; the last pass writes the N0 rows the
@@ -372,9 +383,9 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_col_adj(ColArgs a) {
   {
     const int c = tid % C, t = tid / C;
     float2 v[16];
-    pass_load<NC, 16, LS>(v, tile + c, t);
+    pass_load<NC, 16, ColFftLay<C>>(v, tile + c, t);
     __syncthreads();
-    fft_regs<NC, LS, 1>(v, tile + c, t, Tw);
+    fft_regs<NC, ColFftLay<C>, 1>(v, tile + c, t, Tw);
     const int h0 = a.N0 >> 1;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -479,7 +490,7 @@ __global__ __launch_bounds__(1024) void k_combine(CombineArgs a) {
 template <int NR, int RPB>
 __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
   constexpr int T = NR / 16;
-  __shared__ float2 lds[RPB * NR];
+  __shared__ float2 lds[RPB * kRowLds<NR>];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int row = blockIdx.x * RPB + team;
@@ -504,8 +515,8 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
     if (has_data && X < a.N0) gv = a.sino[(size_t)in_row * a.N0 + X] / a.ck1[X];
     v[r] = make_float2(gv, 0.f);
   }
-  float2* L = lds + team * NR;
-  fft_regs<NR, 1, -1>(v, L, t, a.tw);
+  float2* L = lds + team * kRowLds<NR>;
+  fft_regs<NR, RowLay, -1>(v, L, t, a.tw);
 #pragma unroll
   for (int i = 0; i < 16; i++) L[out_index<NR>(t, i)] = v[i];
   __syncthreads();
@@ -532,7 +543,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
 
 template <int NR>
 __global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
-  __shared__ float2 lds[NR];
+  __shared__ float2 lds[kRowLds<NR>];
   const int s = blockIdx.x;
   const int t = threadIdx.x;
   const float2* f = a.frep + (size_t)s * a.N0;
@@ -563,7 +574,7 @@ __global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
     }
     v[r] = make_float2(ax, ay);
   }
-  fft_regs<NR, 1, 1>(v, lds, t, a.tw);
+  fft_regs<NR, RowLay, 1>(v, lds, t, a.tw);
   const int h = a.N0 >> 1;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
@@ -579,7 +590,7 @@ template <int NR, int RPB>
 __global__ __launch_bounds__(RPB * (NR / 16)) void k_fft_rows(float2* data, int rows, int sign,
                                                              const float2* tw) {
   constexpr int T = NR / 16;
-  __shared__ float2 lds[RPB * NR];
+  __shared__ float2 lds[RPB * kRowLds<NR>];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int row = blockIdx.x * RPB + team;
@@ -588,8 +599,8 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_fft_rows(float2* data, int 
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) v[r] = valid ? p[in_index<NR>(t, r)] : make_float2(0.f, 0.f);
-  if (sign < 0) fft_regs<NR, 1, -1>(v, lds + team * NR, t, tw);
-  else fft_regs<NR, 1, 1>(v, lds + team * NR, t, tw);
+  if (sign < 0) fft_regs<NR, RowLay, -1>(v, lds + team * kRowLds<NR>, t, tw);
+  else fft_regs<NR, RowLay, 1>(v, lds + team * kRowLds<NR>, t, tw);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 16; i++) p[out_index<NR>(t, i)] = v[i];
@@ -609,8 +620,8 @@ __global__ __launch_bounds__(C * (NC / 16)) void k_fft_cols(float2* data, int n1
   constexpr int LS = C + 1;
   for (int i = tid; i < NC * C; i += NT) tile[(i / C) * LS + i % C] = base[(size_t)(i / C) * n1 + y0 + i % C];
   __syncthreads();
-  if (sign < 0) fft_lds<NC, LS, -1>(tile + (tid % C), tid / C, tw);
-  else fft_lds<NC, LS, 1>(tile + (tid % C), tid / C, tw);
+  if (sign < 0) fft_lds<NC, Lay<LS>, -1>(tile + (tid % C), tid / C, tw);
+  else fft_lds<NC, Lay<LS>, 1>(tile + (tid % C), tid / C, tw);
   __syncthreads();
   for (int i = tid; i < NC * C; i += NT) base[(size_t)(i / C) * n1 + y0 + i % C] = tile[(i / C) * LS + i % C];
   (void)count;
@@ -750,10 +761,10 @@ static hipError_t col_attr() {
   static bool done = false;
   if (!done) {
     hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       NC * col_stride<NC, C>() * (int)sizeof(float2));
+                                       col_lds<NC, C>() * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_col_adj<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            NC * col_stride<NC, C>() * (int)sizeof(float2));
+                            col_lds<NC, C>() * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_fft_cols<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             NC * (C + 1) * (int)sizeof(float2));
@@ -780,10 +791,10 @@ hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t
     e = col_attr<N, CC>();                                                                               \
     if (e != hipSuccess) return e;                                                                       \
     if (adjoint)                                                                                         \
-      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (col_stride<N, CC>()) * sizeof(float2), \
+      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), (col_lds<N, CC>()) * sizeof(float2), \
                          st, a);                                                                         \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), N * (col_stride<N, CC>()) * sizeof(float2), \
+      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), (col_lds<N, CC>()) * sizeof(float2), \
                          st, a);                                                                         \
     return hipGetLastError();                                                                            \
   }
