@@ -239,18 +239,27 @@ void build_bins(const Geometry& g, int C, Bins& b) {
     }
   }
   // self-contained records: the node's x-taps, the y-tap under every column
-  // of the tile (zero outside the run) and the node -> entries map
+  // of the tile (zero outside the run) and the node -> entries map.  Within a
+  // tile the records are interleaved by field (x-tap pairs, y-tap quads) so
+  // that consecutive entries of one field are contiguous.
   const size_t E = b.ent.size();
   b.px.assign(E * kTaps, 0.f);
   b.py.assign(E * (size_t)C, 0.f);
   b.of_node.assign((size_t)g.M * kMaxSlot, -1);
-  for (size_t e = 0; e < E; e++) {
-    const int4 r = b.ent[e];
-    const int j = r.x, t0 = r.y & 0xff, t1 = (r.y >> 8) & 0xff, slot = (r.y >> 16) & 0xff;
-    const int col0 = (r.y >> 24) & 0xff;
-    for (int t = 0; t < kTaps; t++) b.px[e * kTaps + t] = g.phix[(size_t)j * kTaps + t];
-    for (int t = t0; t <= t1; t++) b.py[e * C + col0 + t - t0] = g.phiy[(size_t)j * kTaps + t];
-    b.of_node[(size_t)j * kMaxSlot + slot] = (int)e;
+  for (int blk = 0; blk < b.nblk; blk++) {
+    const size_t e0 = b.off[blk], ne = b.off[blk + 1] - e0;
+    for (size_t e = e0; e < e0 + ne; e++) {
+      const int4 r = b.ent[e];
+      const int j = r.x, t0 = r.y & 0xff, t1 = (r.y >> 8) & 0xff, slot = (r.y >> 16) & 0xff;
+      const int col0 = (r.y >> 24) & 0xff;
+      for (int t = 0; t < kTaps; t++)
+        b.px[e0 * kTaps + ((t / 2) * ne + (e - e0)) * 2 + (t & 1)] = g.phix[(size_t)j * kTaps + t];
+      for (int t = t0; t <= t1; t++) {
+        const int c = col0 + t - t0;
+        b.py[e0 * C + ((c / 4) * ne + (e - e0)) * 4 + (c & 3)] = g.phiy[(size_t)j * kTaps + t];
+      }
+      b.of_node[(size_t)j * kMaxSlot + slot] = (int)e;
+    }
   }
 }
 

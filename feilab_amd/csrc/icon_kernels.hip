@@ -158,6 +158,19 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
 // thread, summing its 14 x C cells of the tile.
 // Spread: see k_col_adj (row-owner gather, no atomics).
 
+// A tile's bin records are interleaved by field so that a wavefront reading
+// one field of consecutive entries reads contiguous memory: for the tile's
+// entries [e0, e1), x-tap pair q of entry e is float2 number q*(e1-e0) + e-e0
+// of the block at float e0*14, and y-tap quad c4 is float4 number
+// c4*(e1-e0) + e-e0 of the block at float e0*C.
+__device__ __forceinline__ const float2* tile_px(const ColArgs& a, int e0) {
+  return reinterpret_cast<const float2*>(a.ent_px + (size_t)e0 * kTapsK);
+}
+template <int C>
+__device__ __forceinline__ const float4* tile_py(const ColArgs& a, int e0) {
+  return reinterpret_cast<const float4*>(a.ent_py + (size_t)e0 * C);
+}
+
 // Row stride (float2) of the LDS tile: C + 2 keeps rows 16-byte aligned for
 // vector reads while two blocks still share a CU's LDS; otherwise C + 1.
 template <int NC, int C>
@@ -215,22 +228,22 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   // interpolation partials of this tile's nodes: one bin entry per thread;
   // the entry's record (first x-tap, 14 x-taps, C y-taps) comes in with
   // vector loads and the thread sums its 14 x C cells from the LDS tile
-  const int e1 = a.ent_off[blk + 1];
-  for (int e = a.ent_off[blk] + tid; e < e1; e += NT) {
+  const int e0 = a.ent_off[blk], e1 = a.ent_off[blk + 1], ne = e1 - e0;
+  for (int e = e0 + tid; e < e1; e += NT) {
     const int4 en = a.ent[e];
     float px[kTapsK];
-    const float2* pxv = reinterpret_cast<const float2*>(a.ent_px + (size_t)e * kTapsK);
+    const float2* pxv = tile_px(a, e0) + (e - e0);
 #pragma unroll
     for (int q = 0; q < kTapsK / 2; q++) {
-      const float2 p = pxv[q];
+      const float2 p = pxv[q * ne];
       px[2 * q] = p.x;
       px[2 * q + 1] = p.y;
     }
     float py[C];
-    const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
+    const float4* pyv = tile_py<C>(a, e0) + (e - e0);
 #pragma unroll
     for (int c4 = 0; c4 < C / 4; c4++) {
-      const float4 v = pyv[c4];
+      const float4 v = pyv[c4 * ne];
       py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
     }
     // separable window: each grid row is first reduced over its C y-taps,
@@ -338,21 +351,21 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
     for (int i = tid; i < cn; i += NT) {
       const int e = cb + i;
       float4* rec = reinterpret_cast<float4*>(stage + i * REC);
-      const float2* pxv = reinterpret_cast<const float2*>(a.ent_px + (size_t)e * kTapsK);
+      const float2* pxv = tile_px(a, e0) + (e - e0);
       float t[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) t[q] = 0.f;
 #pragma unroll
       for (int q = 0; q < kTapsK / 2; q++) {
-        const float2 v = pxv[q];
+        const float2 v = pxv[q * (e1 - e0)];
         t[RL - 1 + 2 * q] = v.x;
         t[RL + 2 * q] = v.y;
       }
 #pragma unroll
       for (int q = 0; q < 4; q++) rec[q] = make_float4(t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]);
-      const float4* pyv = reinterpret_cast<const float4*>(a.ent_py + (size_t)e * C);
+      const float4* pyv = tile_py<C>(a, e0) + (e - e0);
 #pragma unroll
-      for (int c4 = 0; c4 < C / 4; c4++) rec[4 + c4] = pyv[c4];
+      for (int c4 = 0; c4 < C / 4; c4++) rec[4 + c4] = pyv[c4 * (e1 - e0)];
       const float2 f = fw[e];
       rec[4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
     }

@@ -42,8 +42,8 @@ struct Bins {
   std::vector<int4> ent;
   std::vector<int> xoff;          // nblk*(n0+1): first entry of the tile with x0 >= x
   std::vector<int8_t> nslot;      // M
-  std::vector<float> px;          // E*14: x-taps of the entry's node
-  std::vector<float> py;          // E*C: y-tap of each tile column (0 outside the run)
+  std::vector<float> px;          // E*14: x-taps of the entry's node (interleaved per tile)
+  std::vector<float> py;          // E*C: y-tap of each tile column, 0 outside the run (interleaved per tile)
   std::vector<int> of_node;       // M*3: entries of node j by slot (-1: none)
 };
 
