@@ -85,14 +85,14 @@ int upload(icon_plan* p, T** ptr, const std::vector<T>& h) {
 }
 
 void free_bins(icon_plan* p) {
-  if (p->d_ent_off) hipFree(p->d_ent_off);
-  if (p->d_ent) hipFree(p->d_ent);
-  if (p->d_nslot) hipFree(p->d_nslot);
-  if (p->d_xoff) hipFree(p->d_xoff);
-  if (p->d_ent_px) hipFree(p->d_ent_px);
-  if (p->d_ent_py) hipFree(p->d_ent_py);
-  if (p->d_ent_of_node) hipFree(p->d_ent_of_node);
-  if (p->d_fw) hipFree(p->d_fw);
+  if (p->d_ent_off) (void)hipFree(p->d_ent_off);
+  if (p->d_ent) (void)hipFree(p->d_ent);
+  if (p->d_nslot) (void)hipFree(p->d_nslot);
+  if (p->d_xoff) (void)hipFree(p->d_xoff);
+  if (p->d_ent_px) (void)hipFree(p->d_ent_px);
+  if (p->d_ent_py) (void)hipFree(p->d_ent_py);
+  if (p->d_ent_of_node) (void)hipFree(p->d_ent_of_node);
+  if (p->d_fw) (void)hipFree(p->d_fw);
   p->d_ent_off = nullptr; p->d_ent = nullptr; p->d_nslot = nullptr; p->d_xoff = nullptr;
   p->d_ent_px = nullptr; p->d_ent_py = nullptr; p->d_ent_of_node = nullptr; p->d_fw = nullptr;
 }
@@ -113,7 +113,7 @@ int upload_bins(icon_plan* p) {
 
 void drop_graph(icon_plan* p) {
   for (int i = 0; i < kGraphCache; i++) {
-    if (p->graph_exec[i]) hipGraphExecDestroy(p->graph_exec[i]);
+    if (p->graph_exec[i]) (void)hipGraphExecDestroy(p->graph_exec[i]);
     p->graph_exec[i] = nullptr;
     p->graph_key[i] = GraphKey{};
     p->graph_used[i] = 0;
@@ -154,6 +154,7 @@ int op_forward(icon_plan* p, int S, const void* img, int cplx, float2* out, hipS
   int rc = check_launch(launch_row_fwd(g.n1, ra, st), "row_fwd");
   if (rc) return rc;
   ColArgs ca{};
+  ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
   ca.in = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.part = p->d_part;
@@ -169,6 +170,7 @@ int op_adjoint_ent(icon_plan* p, int S, float2* img, hipStream_t st) {
   const Geometry& g = p->g;
   const float2* f = p->d_fw;
   ColArgs ca{};
+  ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
   ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.fw = f;
@@ -191,6 +193,7 @@ int op_adjoint(icon_plan* p, int S, const float2* f, float2* img, hipStream_t st
 int op_sino(icon_plan* p, int S, const float* sino, float2* b, hipStream_t st) {
   const Geometry& g = p->g;
   SinoArgs sa{};
+  sa.K = p->g.K;
   sa.rows = S * g.nz; sa.N0 = g.N0; sa.nz = g.nz; sa.skip = g.cv_skip; sa.sino = sino; sa.ck1 = p->d_ck1;
   sa.phi1 = p->d_phi1;
   sa.i10 = p->d_i10; sa.tw = p->d_tw0; sa.b = b;
@@ -225,6 +228,7 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.data_type = o.data_type; rf.thr = o.threshold;
   rf.m = m; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha; rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
   ColArgs ca{};
+  ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
   ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
@@ -275,6 +279,7 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
     if ((rc = check_launch(launch_combine(cm, st), "combine_final"))) return rc;
     if (reproj) {
       ReprojArgs rp{};
+      rp.K = p->g.K;
       rp.S = S; rp.N0 = g.N0; rp.sigma = g.sig0; rp.frep = p->d_frep; rp.ck1 = p->d_ck1;
       rp.phi1 = p->d_phi1; rp.i10 = p->d_i10; rp.tw = p->d_tw0; rp.out = reproj;
       if ((rc = check_launch(launch_reproj(g.n0, rp, st), "reproj"))) return rc;
@@ -283,10 +288,11 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   return ICON_OK;
 }
 
-int prepare_col(int n0, int C) {
+int prepare_col(int n0, int C, int K) {
   // sets the LDS attribute outside any capture
-  if (!col_config_ok(n0, C)) return fail(ICON_ERR_ARG, "column tile %d unsupported for grid %d", C, n0);
-  hipError_t e = set_col_attrs(n0, C);
+  if (!col_config_ok(n0, C, K))
+    return fail(ICON_ERR_ARG, "column tile %d unsupported for grid %d with %d window taps", C, n0, K);
+  hipError_t e = set_col_attrs(n0, C, K);
   if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "hipFuncSetAttribute: %s", hipGetErrorString(e));
   return ICON_OK;
 }
@@ -361,7 +367,7 @@ int icon_geometry_tables(int nx, int thickness, int nz, const float* tilt_deg, i
   if (info) {
     std::memset(info, 0, sizeof(*info));
     info->N0 = g.N0; info->N1 = g.N1; info->nz = g.nz; info->M = g.M; info->n0 = g.n0; info->n1 = g.n1;
-    info->m = kWindowM; info->iomit = g.iomit; info->sigma0 = g.sig0; info->sigma1 = g.sig1;
+    info->m = g.K == kTaps ? kWindowM : g.K / 2; info->window = g.window; info->iomit = g.iomit; info->sigma0 = g.sig0; info->sigma1 = g.sig1;
     info->b0 = g.b0; info->b1 = g.b1;
   }
   return ICON_OK;
@@ -369,7 +375,7 @@ int icon_geometry_tables(int nx, int thickness, int nz, const float* tilt_deg, i
 
 namespace {
 int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg, int max_slices,
-                bool cv);
+                bool cv, int window = ICON_WINDOW_GAUSS);
 }
 
 int icon_plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg,
@@ -382,11 +388,16 @@ int icon_plan_create_cv(icon_plan** out, int device, int nx, int thickness, int 
   return plan_create(out, device, nx, thickness, nz, tilt_deg, max_slices, true);
 }
 
+int icon_plan_create_window(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg,
+                            int max_slices, int cross_validation, int window) {
+  return plan_create(out, device, nx, thickness, nz, tilt_deg, max_slices, cross_validation != 0, window);
+}
+
 }  // extern "C"
 
 namespace {
 int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, const float* tilt_deg, int max_slices,
-                bool cv) {
+                bool cv, int window) {
   if (!out || !tilt_deg) return fail(ICON_ERR_ARG, "icon_plan_create: null pointer");
   *out = nullptr;
   if (max_slices < 1) return fail(ICON_ERR_ARG, "icon_plan_create: max_slices must be >= 1");
@@ -398,7 +409,7 @@ int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, cons
   auto* p = new icon_plan();
   p->device = device;
   std::string err;
-  if (!build_geometry(p->g, nx, thickness > 0 ? thickness : nx, nz, tilt_deg, &err, cv)) {
+  if (!build_geometry(p->g, nx, thickness > 0 ? thickness : nx, nz, tilt_deg, &err, cv, window)) {
     delete p;
     return fail(ICON_ERR_ARG, "%s", err.c_str());
   }
@@ -442,7 +453,7 @@ int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, cons
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming);
     if (e != hipSuccess) return bail(fail(ICON_ERR_DEVICE, "stream/event create: %s", hipGetErrorString(e)));
   }
-  if ((rc = prepare_col(g.n0, p->bins.C))) return bail(rc);
+  if ((rc = prepare_col(g.n0, p->bins.C, g.K))) return bail(rc);
   *out = p;
   return ICON_OK;
 }
@@ -453,17 +464,17 @@ extern "C" {
 int icon_plan_destroy(icon_plan* p) {
   if (!p) return ICON_OK;
   DevGuard dg(p->device);
-  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   drop_graph(p);
   void* ptrs[] = {p->d_ck, p->d_ck1, p->d_w, p->d_phix, p->d_phiy, p->d_ix0, p->d_phi1, p->d_i10,
                   p->d_b, p->d_part, p->d_ahwb, p->d_rhat, p->d_tmp, p->d_frep, p->d_rowpart,
                   p->d_bnorm, p->d_alpha};
   for (void* q : ptrs)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   free_bins(p);
-  if (p->ev_in) hipEventDestroy(p->ev_in);
-  if (p->ev_out) hipEventDestroy(p->ev_out);
-  if (p->stream) hipStreamDestroy(p->stream);
+  if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+  if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return ICON_OK;
 }
@@ -472,7 +483,7 @@ int icon_plan_get_info(const icon_plan* p, icon_plan_info* info) {
   if (!p || !info) return fail(ICON_ERR_ARG, "icon_plan_get_info: null pointer");
   const Geometry& g = p->g;
   info->N0 = g.N0; info->N1 = g.N1; info->nz = g.nz; info->M = g.M; info->n0 = g.n0; info->n1 = g.n1;
-  info->m = kWindowM; info->iomit = g.iomit; info->sigma0 = g.sig0; info->sigma1 = g.sig1;
+  info->m = g.K == kTaps ? kWindowM : g.K / 2; info->window = g.window; info->iomit = g.iomit; info->sigma0 = g.sig0; info->sigma1 = g.sig1;
   info->b0 = g.b0; info->b1 = g.b1; info->max_slices = p->S_max; info->col_block = p->bins.C;
   info->device_bytes = p->device_bytes;
   return ICON_OK;
@@ -480,7 +491,7 @@ int icon_plan_get_info(const icon_plan* p, icon_plan_info* info) {
 
 int icon_plan_set_col_block(icon_plan* p, int C) {
   if (!p) return fail(ICON_ERR_ARG, "icon_plan_set_col_block: null plan");
-  if (!col_config_ok(p->g.n0, C) || p->g.n1 % C != 0)
+  if (!col_config_ok(p->g.n0, C, p->g.K) || p->g.n1 % C != 0)
     return fail(ICON_ERR_ARG, "column tile %d unsupported for grid %d x %d", C, p->g.n0, p->g.n1);
   DevGuard dg(p->device);
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -489,7 +500,7 @@ int icon_plan_set_col_block(icon_plan* p, int C) {
   build_bins(p->g, C, p->bins);
   int rc = upload_bins(p);
   if (rc) return rc;
-  return prepare_col(p->g.n0, C);
+  return prepare_col(p->g.n0, C, p->g.K);
 }
 
 int icon_nufft_forward_real(icon_plan* p, int S, const void* d_img, void* d_f, void* stream) {
@@ -573,7 +584,7 @@ int icon_reconstruct(icon_plan* p, int S, const void* d_sino, void* d_m, const v
       slot = 0;   // least recently used (empty slots have time 0)
       for (int i = 1; i < kGraphCache; i++)
         if (p->graph_used[i] < p->graph_used[slot]) slot = i;
-      if (p->graph_exec[slot]) hipGraphExecDestroy(p->graph_exec[slot]);
+      if (p->graph_exec[slot]) (void)hipGraphExecDestroy(p->graph_exec[slot]);
       p->graph_exec[slot] = nullptr;
       p->graph_key[slot] = GraphKey{};
       HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
@@ -582,12 +593,12 @@ int icon_reconstruct(icon_plan* p, int S, const void* d_sino, void* d_m, const v
       hipGraph_t graph = nullptr;
       hipError_t e = hipStreamEndCapture(p->stream, &graph);
       if (rc) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
       }
       if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
       e = hipGraphInstantiate(&p->graph_exec[slot], graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
+      (void)hipGraphDestroy(graph);
       if (e != hipSuccess) {
         p->graph_exec[slot] = nullptr;
         return fail(ICON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
@@ -621,7 +632,7 @@ int icon_reconstruct_host(icon_plan* p, int S, const float* sino, float* m, cons
   int rc = ICON_OK;
   auto cleanup = [&]() {
     for (void* q : {ds, dm, dmi, drp, drs, dal})
-      if (q) hipFree(q);
+      if (q) (void)hipFree(q);
   };
 #define TRY(expr)                                              \
   do {                                                         \
@@ -662,6 +673,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.cplx = 1; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
   rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
   ColArgs ca{};
+  ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
   ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
@@ -673,6 +685,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 1; ra.in = p->d_tmp; ra.ck = p->d_ck;
   ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart;
   SinoArgs sa{};
+  sa.K = p->g.K;
   sa.rows = S * g.nz; sa.N0 = g.N0; sa.sino = reinterpret_cast<const float*>(p->d_b); sa.ck1 = p->d_ck1;
   sa.phi1 = p->d_phi1; sa.i10 = p->d_i10; sa.tw = p->d_tw0; sa.b = p->d_fw;
   auto launch = [&]() -> hipError_t {

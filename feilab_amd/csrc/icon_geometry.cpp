@@ -57,6 +57,56 @@ void taps(float c, int N, float sigma, float b, float bf, int m, float* phi, int
     if (t == 0) *first = (l + n) % n;
   }
 }
+// Kaiser-Bessel window of width W = 6 grid points (6 taps per dimension),
+// an alternative to the reference's Gaussian: phi(d) = I0(beta sqrt(1 -
+// (2d/W)^2)) / Z for |d| <= W/2, with beta = pi sqrt((W/sigma)^2 (sigma -
+// 1/2)^2 - 0.8) (Beatty, Nishimura and Pauly, IEEE TMI 24(6), 2005) and
+// Z = W sinh(beta) / beta, so that the deapodisation (its Fourier
+// transform W sinh(a) / a, a = sqrt(beta^2 - (pi W k / n)^2), over Z) is 1
+// at k = 0 like the Gaussian's.  Computed in double, stored in float.
+double bessel_i0(double x) {
+  // power series sum_k ((x/2)^k / k!)^2; |x| <= 20 here
+  double term = 1.0, sum = 1.0;
+  const double q = 0.25 * x * x;
+  for (int k = 1; k < 200 && term > 1e-17 * sum; k++) {
+    term *= q / ((double)k * (double)k);
+    sum += term;
+  }
+  return sum;
+}
+
+float kb_beta(float sigma) {
+  const double W = kTapsKB, s = sigma;
+  return (float)(kPI * std::sqrt((W / s) * (W / s) * (s - 0.5) * (s - 0.5) - 0.8));
+}
+
+double kb_norm(double beta) { return kTapsKB * std::sinh(beta) / beta; }
+
+float deapod_kb(int k, int N, float sigma, float beta) {
+  const double n = std::ceil((float)N * sigma);
+  const double W = kTapsKB, bt = beta;
+  const double u = kPI * W * (double)k / n;
+  const double a2 = bt * bt - u * u;
+  double v;
+  if (a2 > 0) v = W * std::sinh(std::sqrt(a2)) / std::sqrt(a2);
+  else if (a2 < 0) v = W * std::sin(std::sqrt(-a2)) / std::sqrt(-a2);
+  else v = W;
+  return (float)(v / kb_norm(bt));
+}
+
+// Kaiser-Bessel taps of one coordinate: grid points floor(n c) - 2 .. + 3.
+void taps_kb(float c, int N, float sigma, float beta, float* phi, int* first) {
+  const int n = (int)std::ceil((float)N * sigma);
+  const double u = (double)n * (double)c;
+  const int l0 = (int)std::floor(u) - (kTapsKB / 2 - 1);
+  const double Z = kb_norm(beta);
+  for (int t = 0; t < kTapsKB; t++) {
+    const double d = u - (double)(l0 + t);
+    const double r = 1.0 - (2.0 * d / kTapsKB) * (2.0 * d / kTapsKB);
+    phi[t] = r > 0 ? (float)(bessel_i0((double)beta * std::sqrt(r)) / Z) : 0.f;
+  }
+  *first = ((l0 % n) + n) % n;
+}
 }  // namespace
 
 void grid_size(int N, float* sigma, int* n) {
@@ -66,7 +116,12 @@ void grid_size(int N, float* sigma, int* n) {
   *n = (int)std::ceil(s * (float)N);
 }
 
-bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv) {
+bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv,
+                    int window) {
+  if (window != ICON_WINDOW_GAUSS && window != ICON_WINDOW_KB6) {
+    if (err) *err = "icon_plan_create: unknown window";
+    return false;
+  }
   if (N0 < 8 || N1 < 8 || nz < (cv ? 3 : 2)) {
     if (err) *err = cv ? "icon_plan_create_cv: need nx >= 8, thickness >= 8 and at least 3 tilts"
                        : "icon_plan_create: need nx >= 8, thickness >= 8 and at least 2 tilts";
@@ -91,8 +146,17 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
       return false;
     }
   }
-  window_shape(g.sig0, kWindowM, &g.b0, &g.bf0);
-  window_shape(g.sig1, kWindowM, &g.b1, &g.bf1);
+  g.window = window;
+  const bool kb = window == ICON_WINDOW_KB6;
+  g.K = kb ? kTapsKB : kTaps;
+  if (kb) {
+    g.b0 = kb_beta(g.sig0);
+    g.b1 = kb_beta(g.sig1);
+    g.bf0 = g.bf1 = 0.f;
+  } else {
+    window_shape(g.sig0, kWindowM, &g.b0, &g.bf0);
+    window_shape(g.sig1, kWindowM, &g.b1, &g.bf1);
+  }
 
   // angles are negated on read (MainCode.cu:42); the omitted tilt is the
   // first of least |theta| (MainCode.cu:46-49)
@@ -157,46 +221,55 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
   }
 
   // 2D window taps and first indices
-  g.phix.resize((size_t)M * kTaps);
-  g.phiy.resize((size_t)M * kTaps);
+  const int K = g.K;
+  auto node_taps = [&](float c, int N, float sigma, float b, float bf, float* phi, int* first) {
+    if (kb) taps_kb(c, N, sigma, b, phi, first);
+    else taps(c, N, sigma, b, bf, kWindowM, phi, first);
+  };
+  auto deapod_of = [&](int k, int N, float sigma, float b) {
+    return kb ? deapod_kb(k, N, sigma, b) : deapod(k, N, sigma, kWindowM);
+  };
+  g.phix.resize((size_t)M * K);
+  g.phiy.resize((size_t)M * K);
   g.ix0.resize(M);
   g.iy0.resize(M);
   for (int j = 0; j < M; j++) {
-    taps(g.nodes[2 * (size_t)j], N0, g.sig0, g.b0, g.bf0, kWindowM, &g.phix[(size_t)j * kTaps], &g.ix0[j]);
-    taps(g.nodes[2 * (size_t)j + 1], N1, g.sig1, g.b1, g.bf1, kWindowM, &g.phiy[(size_t)j * kTaps], &g.iy0[j]);
+    node_taps(g.nodes[2 * (size_t)j], N0, g.sig0, g.b0, g.bf0, &g.phix[(size_t)j * K], &g.ix0[j]);
+    node_taps(g.nodes[2 * (size_t)j + 1], N1, g.sig1, g.b1, g.bf1, &g.phiy[(size_t)j * K], &g.iy0[j]);
   }
 
   // deapodisation
   g.ck.resize((size_t)N0 * N1);
   {
     std::vector<float> cy(N1);
-    for (int Y = 0; Y < N1; Y++) cy[Y] = deapod(Y - N1 / 2, N1, g.sig1, kWindowM);
+    for (int Y = 0; Y < N1; Y++) cy[Y] = deapod_of(Y - N1 / 2, N1, g.sig1, g.b1);
     for (int X = 0; X < N0; X++) {
-      const float cx = deapod(X - N0 / 2, N0, g.sig0, kWindowM);
+      const float cx = deapod_of(X - N0 / 2, N0, g.sig0, g.b0);
       for (int Y = 0; Y < N1; Y++) g.ck[(size_t)X * N1 + Y] = cx * cy[Y];
     }
   }
 
   // 1D plan of the projection rows: nodes (k - N0/2)/N0, same sigma as dim 0
   g.ck1.resize(N0);
-  g.phi1.resize((size_t)N0 * kTaps);
+  g.phi1.resize((size_t)N0 * K);
   g.i10.resize(N0);
   for (int k = 0; k < N0; k++) {
     const float x = (float)(((double)(float)k - (double)(float)N0 / 2.0) / (double)(float)N0);
-    taps(x, N0, g.sig0, g.b0, g.bf0, kWindowM, &g.phi1[(size_t)k * kTaps], &g.i10[k]);
-    g.ck1[k] = deapod(k - N0 / 2, N0, g.sig0, kWindowM);
+    node_taps(x, N0, g.sig0, g.b0, g.bf0, &g.phi1[(size_t)k * K], &g.i10[k]);
+    g.ck1[k] = deapod_of(k - N0 / 2, N0, g.sig0, g.b0);
   }
   return true;
 }
 
-// Bins every node's run of y-taps by grid-column tile.  A node's 14 taps are
-// consecutive columns (mod n1), so they fall into at most ceil(13/C)+1 tiles;
+// Bins every node's run of y-taps by grid-column tile.  A node's K taps are
+// consecutive columns (mod n1), so they fall into at most ceil((K-1)/C)+1 tiles;
 // each (tile, node) pair is one entry recording the tap range, the node's
 // partial-sum slot, the first column inside the tile and the node's first
 // x-tap.  A tile's entries are sorted by first x-tap (then node), and xoff
 // gives, per tile and grid row x, the first entry whose first x-tap is >= x,
-// so the entries covering row x are those with first x-tap in [x-13, x].
+// so the entries covering row x are those with first x-tap in [x-K+1, x].
 void build_bins(const Geometry& g, int C, Bins& b) {
+  const int K = g.K;
   b.C = C;
   b.nblk = g.n1 / C;
   b.off.assign(b.nblk + 1, 0);
@@ -204,7 +277,7 @@ void build_bins(const Geometry& g, int C, Bins& b) {
   const int mask = g.n1 - 1;
   for (int j = 0; j < g.M; j++) {
     int prev = -1;
-    for (int t = 0; t < kTaps; t++) {
+    for (int t = 0; t < K; t++) {
       const int blk = ((g.iy0[j] + t) & mask) / C;
       if (blk != prev) { b.off[blk + 1]++; prev = blk; }
     }
@@ -214,11 +287,11 @@ void build_bins(const Geometry& g, int C, Bins& b) {
   std::vector<int> fill(b.off.begin(), b.off.end() - 1);
   for (int j = 0; j < g.M; j++) {
     int t = 0, slot = 0;
-    while (t < kTaps) {
+    while (t < K) {
       const int col = (g.iy0[j] + t) & mask;
       const int blk = col / C;
       int t1 = t;
-      while (t1 + 1 < kTaps && (((g.iy0[j] + t1 + 1) & mask) / C) == blk) t1++;
+      while (t1 + 1 < K && (((g.iy0[j] + t1 + 1) & mask) / C) == blk) t1++;
       const int col0 = col - blk * C;
       b.ent[fill[blk]++] = make_int4(j, t | (t1 << 8) | (slot << 16) | (col0 << 24), g.ix0[j], 0);
       slot++;
@@ -243,7 +316,7 @@ void build_bins(const Geometry& g, int C, Bins& b) {
   // tile the records are interleaved by field (x-tap pairs, y-tap quads) so
   // that consecutive entries of one field are contiguous.
   const size_t E = b.ent.size();
-  b.px.assign(E * kTaps, 0.f);
+  b.px.assign(E * K, 0.f);
   b.py.assign(E * (size_t)C, 0.f);
   b.of_node.assign((size_t)g.M * kMaxSlot, -1);
   for (int blk = 0; blk < b.nblk; blk++) {
@@ -252,11 +325,11 @@ void build_bins(const Geometry& g, int C, Bins& b) {
       const int4 r = b.ent[e];
       const int j = r.x, t0 = r.y & 0xff, t1 = (r.y >> 8) & 0xff, slot = (r.y >> 16) & 0xff;
       const int col0 = (r.y >> 24) & 0xff;
-      for (int t = 0; t < kTaps; t++)
-        b.px[e0 * kTaps + ((t / 2) * ne + (e - e0)) * 2 + (t & 1)] = g.phix[(size_t)j * kTaps + t];
+      for (int t = 0; t < K; t++)
+        b.px[e0 * K + ((t / 2) * ne + (e - e0)) * 2 + (t & 1)] = g.phix[(size_t)j * K + t];
       for (int t = t0; t <= t1; t++) {
         const int c = col0 + t - t0;
-        b.py[e0 * C + ((c / 4) * ne + (e - e0)) * 4 + (c & 3)] = g.phiy[(size_t)j * kTaps + t];
+        b.py[e0 * C + ((c / 4) * ne + (e - e0)) * 4 + (c & 3)] = g.phiy[(size_t)j * K + t];
       }
       b.of_node[(size_t)j * kMaxSlot + slot] = (int)e;
     }

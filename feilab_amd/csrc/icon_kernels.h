@@ -6,7 +6,8 @@
 
 namespace icon {
 
-constexpr int kTapsK = 14;
+constexpr int kTapsK = 14;     // Gaussian window taps per dimension
+constexpr int kTapsKBK = 6;    // Kaiser-Bessel window taps per dimension
 constexpr int kMaxSlotK = 3;
 
 struct RowFwdArgs {
@@ -37,6 +38,7 @@ struct RowAdjArgs {
 
 struct ColArgs {
   int S, N0, N1, M, n1, E;
+  int K;                   // window taps per dimension (kTapsK or kTapsKBK)
   const float2* in;        // fwd: tmp [S][N0][n1]
   float2* out;             // adj: tmp [S][N0][n1]
   const float2* tw;
@@ -46,8 +48,8 @@ struct ColArgs {
   const float* phix;
   const float* phiy;
   const int* ix0;
-  const float* ent_px;     // [E][14]
-  const float* ent_py;     // [E][C]
+  const float* ent_px;     // [E][K], interleaved per tile
+  const float* ent_py;     // [E][C], interleaved per tile
   float2* part;            // fwd: [S][M][3]
   const float2* fw;        // adj: [S][E] weighted value per bin entry
 };
@@ -74,6 +76,7 @@ struct CombineArgs {
 
 struct SinoArgs {
   int rows, N0;
+  int K;                 // window taps
   int nz, skip;          // tilts per slice; input row left out (cross validation), else -1
   const float* sino;     // [S][nz][N0]
   const float* ck1;
@@ -85,6 +88,7 @@ struct SinoArgs {
 
 struct ReprojArgs {
   int S, N0;
+  int K;                 // window taps
   float sigma;
   const float2* frep;    // [S][N0]
   const float* ck1;
@@ -108,7 +112,7 @@ hipError_t launch_zero(float* p, size_t n, hipStream_t st);
 hipError_t launch_node_to_ent(const float2* f, float2* fw_ent, const int4* ent, int S, int M, int E, hipStream_t st);
 hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
 hipError_t launch_copy4(float4* dst, const float4* src, size_t n, hipStream_t st);
-bool col_config_ok(int NC, int C);
-hipError_t set_col_attrs(int NC, int C);
+bool col_config_ok(int NC, int C, int K);
+hipError_t set_col_attrs(int NC, int C, int K);
 
 }  // namespace icon

@@ -162,9 +162,10 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
 // one field of consecutive entries reads contiguous memory: for the tile's
 // entries [e0, e1), x-tap pair q of entry e is float2 number q*(e1-e0) + e-e0
 // of the block at float e0*14, and y-tap quad c4 is float4 number
-// c4*(e1-e0) + e-e0 of the block at float e0*C.
+// c4*(e1-e0) + e-e0 of the block at float e0*C (K = window taps).
+template <int K>
 __device__ __forceinline__ const float2* tile_px(const ColArgs& a, int e0) {
-  return reinterpret_cast<const float2*>(a.ent_px + (size_t)e0 * kTapsK);
+  return reinterpret_cast<const float2*>(a.ent_px + (size_t)e0 * K);
 }
 template <int C>
 __device__ __forceinline__ const float4* tile_py(const ColArgs& a, int e0) {
@@ -192,7 +193,7 @@ __host__ __device__ constexpr int col_lds() {
                                                                  : ColFftLay<C>::at(NC - 1) + C;
 }
 
-template <int NC, int C>
+template <int NC, int C, int K>
 __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   constexpr int NT = C * (NC / 16);
   constexpr int LS = col_stride<NC, C>();
@@ -231,10 +232,10 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   const int e0 = a.ent_off[blk], e1 = a.ent_off[blk + 1], ne = e1 - e0;
   for (int e = e0 + tid; e < e1; e += NT) {
     const int4 en = a.ent[e];
-    float px[kTapsK];
-    const float2* pxv = tile_px(a, e0) + (e - e0);
+    float px[K];
+    const float2* pxv = tile_px<K>(a, e0) + (e - e0);
 #pragma unroll
-    for (int q = 0; q < kTapsK / 2; q++) {
+    for (int q = 0; q < K / 2; q++) {
       const float2 p = pxv[q * ne];
       px[2 * q] = p.x;
       px[2 * q + 1] = p.y;
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
     // then weighted by its x-tap
     float ax = 0.f, ay = 0.f;
 #pragma unroll 2
-    for (int q = 0; q < kTapsK; q++) {
+    for (int q = 0; q < K; q++) {
       const float2* row = tile + ((en.z + q) & (NC - 1)) * LS;
       float2 gv[C];
       if constexpr (LS % 2 == 0) {
@@ -277,15 +278,16 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   }
 }
 
-template <int NC, int C>
+template <int NC, int C, int K>
 __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   constexpr int NT = C * (NC / 16);
   constexpr int RL = NC / NT;                    // adjacent grid rows per thread
   static_assert(RL >= 1 && RL <= 2 && NT * RL == NC, "one or two grid rows per thread");
-  // staged record, 16-byte aligned fields: the 14 x-taps at [RL-1, RL+13)
-  // with zeros around them in [0, 16), the y-taps at [16, 16+C), then
+  // staged record, 16-byte aligned fields: the K x-taps at [RL-1, RL-1+K)
+  // with zeros around them in [0, XS), the y-taps at [XS, XS+C), then
   // (f.x, f.y, first x-tap, pad)
-  constexpr int REC = 16 + C + 4;
+  constexpr int XS = (K + RL - 1 + 3) / 4 * 4;
+  constexpr int REC = XS + C + 4;
   constexpr int CAP = col_lds<NC, C>() * 2 / REC;  // records staged at once in the tile's LDS
   extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
   // tile-major order: blocks in flight together work on the same column
@@ -306,11 +308,11 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   float* stage = reinterpret_cast<float*>(tile);
   const int x0 = tid * RL;
   int lo[2], hi[2];
-  if (x0 >= kTapsK - 1) {
+  if (x0 >= K - 1) {
     lo[0] = hi[0] = 0;
-    lo[1] = xo[x0 - (kTapsK - 1)]; hi[1] = xo[x0 + RL];
+    lo[1] = xo[x0 - (K - 1)]; hi[1] = xo[x0 + RL];
   } else {
-    lo[0] = xo[NC - (kTapsK - 1) + x0]; hi[0] = xo[NC];
+    lo[0] = xo[NC - (K - 1) + x0]; hi[0] = xo[NC];
     lo[1] = xo[0]; hi[1] = xo[x0 + RL];
   }
   float2 acc[RL][C];
@@ -321,8 +323,8 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   }
   // adds entry record `rec` to the thread's rows
   auto add = [&](const float* rec) {
-    const float4 tail = reinterpret_cast<const float4*>(rec)[4 + C / 4];
-    // x-tap slot of row x0: (x0 - z) + RL - 1, in [0, 15] by the range
+    const float4 tail = reinterpret_cast<const float4*>(rec)[XS / 4 + C / 4];
+    // x-tap slot of row x0: (x0 - z) + RL - 1, in [0, K + RL - 2] by the range
     const int q = (x0 - __float_as_int(tail.z) + RL - 1) & (NC - 1);
     float px[RL];
 #pragma unroll
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
     float py[C];
 #pragma unroll
     for (int c4 = 0; c4 < C / 4; c4++) {
-      const float4 v = reinterpret_cast<const float4*>(rec)[4 + c4];
+      const float4 v = reinterpret_cast<const float4*>(rec)[XS / 4 + c4];
       py[4 * c4] = v.x; py[4 * c4 + 1] = v.y; py[4 * c4 + 2] = v.z; py[4 * c4 + 3] = v.w;
     }
 #pragma unroll
@@ -351,23 +353,23 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
     for (int i = tid; i < cn; i += NT) {
       const int e = cb + i;
       float4* rec = reinterpret_cast<float4*>(stage + i * REC);
-      const float2* pxv = tile_px(a, e0) + (e - e0);
-      float t[16];
+      const float2* pxv = tile_px<K>(a, e0) + (e - e0);
+      float t[XS];
 #pragma unroll
-      for (int q = 0; q < 16; q++) t[q] = 0.f;
+      for (int q = 0; q < XS; q++) t[q] = 0.f;
 #pragma unroll
-      for (int q = 0; q < kTapsK / 2; q++) {
+      for (int q = 0; q < K / 2; q++) {
         const float2 v = pxv[q * (e1 - e0)];
         t[RL - 1 + 2 * q] = v.x;
         t[RL + 2 * q] = v.y;
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) rec[q] = make_float4(t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]);
+      for (int q = 0; q < XS / 4; q++) rec[q] = make_float4(t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]);
       const float4* pyv = tile_py<C>(a, e0) + (e - e0);
 #pragma unroll
-      for (int c4 = 0; c4 < C / 4; c4++) rec[4 + c4] = pyv[c4 * (e1 - e0)];
+      for (int c4 = 0; c4 < C / 4; c4++) rec[XS / 4 + c4] = pyv[c4 * (e1 - e0)];
       const float2 f = fw[e];
-      rec[4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
+      rec[XS / 4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
     }
     __syncthreads();
 #pragma unroll
@@ -538,8 +540,8 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
       float ax = 0.f, ay = 0.f;
       const int i0 = a.i10[k];
 #pragma unroll
-      for (int q = 0; q < kTapsK; q++) {
-        const float ph = a.phi1[(size_t)k * kTapsK + q];
+      for (int q = 0; q < a.K; q++) {
+        const float ph = a.phi1[(size_t)k * a.K + q];
         const float2 gv = L[(i0 + q) & (NR - 1)];
         ax = __fmaf_rn(gv.x, ph, ax);
         ay = __fmaf_rn(gv.y, ph, ay);
@@ -577,8 +579,8 @@ __global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
       const int klo = max(max(kc - 10, 0), kprev + 1), khi = min(kc + 10, a.N0 - 1);
       for (int k = klo; k <= khi; k++) {
         const int d = (x - a.i10[k]) & (NR - 1);
-        if (d < kTapsK) {
-          const float ph = a.phi1[(size_t)k * kTapsK + d];
+        if (d < a.K) {
+          const float ph = a.phi1[(size_t)k * a.K + d];
           ax = __fmaf_rn(f[k].x, ph, ax);
           ay = __fmaf_rn(f[k].y, ph, ay);
         }
@@ -760,36 +762,50 @@ hipError_t launch_fft_rows(int NR, float2* data, int rows, int sign, const float
 }
 
 // Column kernels: (NC, C) with NC * C * 8 bytes of LDS.
-#define COL_CASES(X) X(128, 8) X(128, 16) X(256, 8) X(256, 16) X(512, 8) X(512, 16) X(1024, 8) X(1024, 16) X(2048, 8)
+// (n0, C, taps) of the column kernels built: the Gaussian window at every
+// tile width, the Kaiser-Bessel window at C = 8
+#define COL_CASES(X)                                                                                 \
+  X(128, 8, 14) X(128, 16, 14) X(256, 8, 14) X(256, 16, 14) X(512, 8, 14) X(512, 16, 14) X(1024, 8, 14) \
+  X(1024, 16, 14) X(2048, 8, 14) X(128, 8, 6) X(256, 8, 6) X(512, 8, 6) X(1024, 8, 6) X(2048, 8, 6)
+#define FFT_COL_CASES(X) X(128, 8) X(128, 16) X(256, 8) X(256, 16) X(512, 8) X(512, 16) X(1024, 8) X(1024, 16) X(2048, 8)
 
-bool col_config_ok(int NC, int C) {
-#define CASE(N, CC) if (NC == N && C == CC) return true;
+bool col_config_ok(int NC, int C, int K) {
+#define CASE(N, CC, KK) if (NC == N && C == CC && K == KK) return true;
   COL_CASES(CASE)
 #undef CASE
   return false;
 }
 
-template <int NC, int C>
+template <int NC, int C, int K>
 static hipError_t col_attr() {
   static bool done = false;
   if (!done) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd<NC, C, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        col_lds<NC, C>() * (int)sizeof(float2));
     if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_col_adj<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    e = hipFuncSetAttribute((const void*)k_col_adj<NC, C, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             col_lds<NC, C>() * (int)sizeof(float2));
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_fft_cols<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            NC * (C + 1) * (int)sizeof(float2));
     if (e != hipSuccess) return e;
     done = true;
   }
   return hipSuccess;
 }
 
-hipError_t set_col_attrs(int NC, int C) {
-#define CASE(N, CC) \
-  if (NC == N && C == CC) return col_attr<N, CC>();
+template <int NC, int C>
+static hipError_t fft_col_attr() {
+  static bool done = false;
+  if (!done) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_fft_cols<NC, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       NC * (C + 1) * (int)sizeof(float2));
+    if (e != hipSuccess) return e;
+    done = true;
+  }
+  return hipSuccess;
+}
+
+hipError_t set_col_attrs(int NC, int C, int K) {
+#define CASE(N, CC, KK) \
+  if (NC == N && C == CC && K == KK) return col_attr<N, CC, KK>();
   COL_CASES(CASE)
 #undef CASE
   return hipErrorInvalidValue;
@@ -799,16 +815,16 @@ hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t
   const ColArgs& h = a;
   const int nblk = h.n1 / C;
   hipError_t e = hipErrorInvalidValue;
-#define CASE(N, CC)                                                                                      \
-  if (NC == N && C == CC) {                                                                              \
-    e = col_attr<N, CC>();                                                                               \
+#define CASE(N, CC, KK)                                                                                  \
+  if (NC == N && C == CC && h.K == KK) {                                                                 \
+    e = col_attr<N, CC, KK>();                                                                           \
     if (e != hipSuccess) return e;                                                                       \
     if (adjoint)                                                                                         \
-      hipLaunchKernelGGL((k_col_adj<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), (col_lds<N, CC>()) * sizeof(float2), \
-                         st, a);                                                                         \
+      hipLaunchKernelGGL((k_col_adj<N, CC, KK>), dim3(h.S * nblk), dim3(CC * (N / 16)),                  \
+                         (col_lds<N, CC>()) * sizeof(float2), st, a);                                    \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_col_fwd<N, CC>), dim3(h.S * nblk), dim3(CC * (N / 16)), (col_lds<N, CC>()) * sizeof(float2), \
-                         st, a);                                                                         \
+      hipLaunchKernelGGL((k_col_fwd<N, CC, KK>), dim3(h.S * nblk), dim3(CC * (N / 16)),                  \
+                         (col_lds<N, CC>()) * sizeof(float2), st, a);                                    \
     return hipGetLastError();                                                                            \
   }
   COL_CASES(CASE)
@@ -821,13 +837,13 @@ hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int s
   const int nblk = n1 / C;
 #define CASE(N, CC)                                                                                     \
   if (NC == N && C == CC) {                                                                             \
-    hipError_t e = col_attr<N, CC>();                                                                   \
+    hipError_t e = fft_col_attr<N, CC>();                                                                   \
     if (e != hipSuccess) return e;                                                                      \
     hipLaunchKernelGGL((k_fft_cols<N, CC>), dim3(count * nblk), dim3(CC * (N / 16)), N * (CC + 1) * sizeof(float2), \
                        st, data, n1, count, sign, tw);                                                  \
     return hipGetLastError();                                                                           \
   }
-  COL_CASES(CASE)
+  FFT_COL_CASES(CASE)
 #undef CASE
   return hipErrorInvalidValue;
 }

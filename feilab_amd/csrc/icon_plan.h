@@ -9,14 +9,18 @@
 
 namespace icon {
 
-constexpr int kTaps = 14;      // 2m+2, m = 6 (nufft_gpu_v8.cuh:5)
+constexpr int kTaps = 14;      // Gaussian window: 2m+2, m = 6 (nufft_gpu_v8.cuh:5)
 constexpr int kWindowM = 6;
+constexpr int kTapsKB = 6;     // Kaiser-Bessel window of width 6
 constexpr int kMaxSlot = 3;    // node y-taps split over at most 3 column tiles (tile >= 8)
 
 // Host-side geometry of one tilt series, computed exactly as the reference's
 // setup does (float/double placement included).
 struct Geometry {
   int N0 = 0, N1 = 0, nz = 0, M = 0, n0 = 0, n1 = 0;
+  int window = ICON_WINDOW_GAUSS;  // ICON_WINDOW_GAUSS or ICON_WINDOW_KB6
+  int K = kTaps;                  // taps per dimension: 14 (Gaussian) or 6 (Kaiser-Bessel)
+  // window shape per dimension: Gaussian b and bf, or Kaiser-Bessel beta (b)
   float sig0 = 0, sig1 = 0, b0 = 0, b1 = 0, bf0 = 0, bf1 = 0;
   int iomit = 0;                  // tilt whose nodes give the reprojection
   float omit = 0;
@@ -26,9 +30,9 @@ struct Geometry {
   std::vector<float> w;           // M density weights
   std::vector<float> ck;          // N0*N1 deapodisation, index X*N1+Y
   std::vector<float> ck1;         // N0
-  std::vector<float> phix, phiy;  // M*14 window taps
+  std::vector<float> phix, phiy;  // M*K window taps
   std::vector<int> ix0, iy0;      // M: wrapped first tap index
-  std::vector<float> phi1;        // N0*14 (1D plan)
+  std::vector<float> phi1;        // N0*K (1D plan)
   std::vector<int> i10;           // N0
 };
 
@@ -48,7 +52,8 @@ struct Bins {
 };
 
 // Fills g; returns false (with a message) on an invalid geometry.
-bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv = false);
+bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv = false,
+                    int window = ICON_WINDOW_GAUSS);
 void build_bins(const Geometry& g, int C, Bins& b);
 void grid_size(int N, float* sigma, int* n);
 

@@ -37,7 +37,11 @@ class PlanInfo(ctypes.Structure):
                 ("n0", ctypes.c_int32), ("n1", ctypes.c_int32), ("m", ctypes.c_int32), ("iomit", ctypes.c_int32),
                 ("sigma0", ctypes.c_float), ("sigma1", ctypes.c_float), ("b0", ctypes.c_float),
                 ("b1", ctypes.c_float), ("max_slices", ctypes.c_int32), ("col_block", ctypes.c_int32),
-                ("device_bytes", ctypes.c_int64)]
+                ("window", ctypes.c_int32), ("device_bytes", ctypes.c_int64)]
+
+
+# icon_window
+WINDOWS = {"gauss": 0, "kb6": 1}
 
 
 class SolveOptions(ctypes.Structure):
@@ -79,6 +83,7 @@ def lib():
         "icon_device_synchronize": [],
         "icon_plan_create": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
         "icon_plan_create_cv": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
+        "icon_plan_create_window": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci, ci, ci],
         "icon_plan_destroy": [vp],
         "icon_plan_get_info": [vp, ctypes.POINTER(PlanInfo)],
         "icon_plan_set_col_block": [vp, ci],
@@ -194,15 +199,22 @@ class Plan:
     node vectors [S][nz*nx] complex64, sinograms [S][nz][nx] float32.
     """
 
-    def __init__(self, nx, tilt_deg, thickness=0, max_slices=1, device=0, cross_validation=False):
+    def __init__(self, nx, tilt_deg, thickness=0, max_slices=1, device=0, cross_validation=False,
+                 window="gauss"):
         """cross_validation=True: the plan of Lib_2D_crossV2 (the least-|angle|
-        tilt left out of the data and used as the reprojection angle)."""
+        tilt left out of the data and used as the reprojection angle).
+        window: "gauss" (the reference's 14-tap Gaussian) or "kb6" (6-tap
+        Kaiser-Bessel)."""
         tilt = np.ascontiguousarray(tilt_deg, np.float32)
         self.h = ctypes.c_void_p()
-        create = lib().icon_plan_create_cv if cross_validation else lib().icon_plan_create
-        _check(create(ctypes.byref(self.h), device, int(nx), int(thickness), len(tilt),
-                      tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(max_slices)),
-               "icon_plan_create")
+        tp = tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        if window == "gauss":
+            create = lib().icon_plan_create_cv if cross_validation else lib().icon_plan_create
+            rc = create(ctypes.byref(self.h), device, int(nx), int(thickness), len(tilt), tp, int(max_slices))
+        else:
+            rc = lib().icon_plan_create_window(ctypes.byref(self.h), device, int(nx), int(thickness), len(tilt), tp,
+                                               int(max_slices), int(bool(cross_validation)), WINDOWS[window])
+        _check(rc, "icon_plan_create")
         self.cross_validation = cross_validation
         info = PlanInfo()
         _check(lib().icon_plan_get_info(self.h, ctypes.byref(info)), "icon_plan_get_info")

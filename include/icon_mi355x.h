@@ -38,6 +38,17 @@ typedef enum {
   ICON_ERR_STATE = 5     /* call out of order (e.g. plan not prepared) */
 } icon_status;
 
+/* NUFFT window of a plan.  ICON_WINDOW_GAUSS is the reference's Gaussian,
+ * m = 6, 14 taps per dimension (nufft_gpu_v8.cu:221-235); icon_plan_create
+ * and icon_plan_create_cv always use it.  ICON_WINDOW_KB6 is a Kaiser-Bessel
+ * window of width 6 (6 taps per dimension, beta from the oversampling
+ * factor): about 5e-6 relative error per transform against the exact
+ * non-uniform DFT, for about a third of the gridding work. */
+typedef enum {
+  ICON_WINDOW_GAUSS = 0,
+  ICON_WINDOW_KB6 = 1
+} icon_window;
+
 typedef struct icon_plan icon_plan;
 
 /* Geometry of a plan, as derived by the reference's setup
@@ -47,12 +58,14 @@ typedef struct {
   int32_t nz;            /* number of tilts */
   int32_t M;             /* nodes = N0 * nz */
   int32_t n0, n1;        /* oversampled grid, n = 2 * nextpow2(N) */
-  int32_t m;             /* window half-width, 6 (14 taps) */
+  int32_t m;             /* window half-width: 6 for the Gaussian (2m+2 = 14 taps),
+                            3 for Kaiser-Bessel (2m = 6 taps) */
   int32_t iomit;         /* first tilt of minimal |theta| (MainCode.cu:46-49) */
   float sigma0, sigma1;  /* oversampling factors, float as the reference */
-  float b0, b1;          /* Gaussian window shape parameters */
+  float b0, b1;          /* window shape: Gaussian b, or Kaiser-Bessel beta */
   int32_t max_slices;    /* batch capacity */
   int32_t col_block;     /* grid columns per spread/interp tile */
+  int32_t window;        /* icon_window */
   int64_t device_bytes;  /* device memory held by the plan */
 } icon_plan_info;
 
@@ -98,6 +111,11 @@ int icon_plan_create(icon_plan** plan, int device, int nx, int thickness, int nz
  * holds the cross-validation reprojections. */
 int icon_plan_create_cv(icon_plan** plan, int device, int nx, int thickness, int nz,
                         const float* tilt_deg, int max_slices);
+/* icon_plan_create / icon_plan_create_cv (cross_validation != 0) with the
+ * NUFFT window chosen (icon_window).  No counterpart in the reference, which
+ * has only the Gaussian. */
+int icon_plan_create_window(icon_plan** plan, int device, int nx, int thickness, int nz,
+                            const float* tilt_deg, int max_slices, int cross_validation, int window);
 int icon_plan_destroy(icon_plan* plan);
 int icon_plan_get_info(const icon_plan* plan, icon_plan_info* info);
 /* Tune the spread/interp column tile (8 or 16 grid columns); rebuilds the
