@@ -55,14 +55,39 @@ int twiddles(int n, const float2** out) {
   auto key = std::make_pair(dev, n);
   auto it = g_tw.find(key);
   if (it != g_tw.end()) { *out = it->second; return ICON_OK; }
-  std::vector<float2> h(n);
+  // exp(-2 pi i q / n) in double, rounded to float
+  std::vector<float2> w(n);
   for (int q = 0; q < n; q++) {
     const double a = -2.0 * M_PI * (double)q / (double)n;
-    h[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    w[q] = make_float2((float)std::cos(a), (float)std::sin(a));
   }
+  // layout read by load_twiddles (icon_fft.h): [dir][slot / 2][t][slot % 2]
+  const int TH = n / 16;
+  std::vector<float2> h((size_t)2 * TH * 32, make_float2(0.f, 0.f));
+  for (int d = 0; d < 2; d++)
+    for (int t = 0; t < TH; t++) {
+      int ns = 1, P = 0;
+      while (ns < n) {
+        const int rp = n / ns >= 16 ? 16 : n / ns;
+        if (ns > 1) {
+          const int step = n / (ns * rp);
+          for (int q = 0; q < 16 / rp; q++) {
+            const int k = (t + q * TH) & (ns - 1);
+            for (int r = 1; r < rp; r++) {
+              float2 v = w[(size_t)r * k * step];
+              if (d) v.y = -v.y;
+              const int slot = P * 16 + q * (rp - 1) + r - 1;
+              h[(((size_t)d * 16 + slot / 2) * TH + t) * 2 + slot % 2] = v;
+            }
+          }
+          P++;
+        }
+        ns *= rp;
+      }
+    }
   float2* d = nullptr;
-  HIPCHK(hipMalloc(&d, sizeof(float2) * n));
-  HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&d, sizeof(float2) * h.size()));
+  HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
   g_tw[key] = d;
   *out = d;
   return ICON_OK;

@@ -142,31 +142,36 @@ struct FftTwiddles {
   float2 w[2][16];
 };
 
-template <int N, int NS, int P, int DIR>
-__device__ __forceinline__ void load_twiddles_from(FftTwiddles& T, int t, const float2* __restrict__ tw) {
-  constexpr int RP = pass_radix<N, NS>();
-  if constexpr (NS > 1) {
-    static_assert(P < 2, "at most two twiddled passes");
-    constexpr int TH = N / 16;
-    constexpr int B = 16 / RP;
-    constexpr int step = N / (NS * RP);
-#pragma unroll
-    for (int q = 0; q < B; q++) {
-      const int k = (t + q * TH) & (NS - 1);
-#pragma unroll
-      for (int r = 1; r < RP; r++) {
-        float2 w = tw[r * k * step];
-        if (DIR > 0) w.y = -w.y;
-        T.w[P][q * (RP - 1) + r - 1] = w;
-      }
-    }
+// Number of passes after the first.
+template <int N>
+__host__ __device__ constexpr int twiddled_passes() {
+  int ns = 1, k = 0;
+  while (ns < N) {
+    if (ns > 1) k++;
+    ns *= N / ns >= 16 ? 16 : N / ns;
   }
-  if constexpr (NS * RP < N) load_twiddles_from<N, NS * RP, (NS > 1 ? P + 1 : 0), DIR>(T, t, tw);
+  return k;
 }
 
+// The twiddle table of size N holds, for direction d (0: DIR = -1, 1: DIR =
+// +1), 32 twiddles per thread t < N/16 as 16 float4 pairs: pair i of thread
+// t is float4 (d * 16 + i) * N/16 + t, and twiddle slot s = 2i + {0, 1} is
+// P*16 + q*(RP-1) + r-1 = exp(DIR 2 pi i r k step / N) of twiddled pass P
+// (sub-transform size NS, radix RP, k = (t + q N/16) mod NS, step =
+// N/(NS RP)).  Consecutive threads read consecutive 16 bytes.
 template <int N, int DIR>
 __device__ __forceinline__ void load_twiddles(FftTwiddles& T, int t, const float2* __restrict__ tw) {
-  load_twiddles_from<N, 1, 0, DIR>(T, t, tw);
+  constexpr int TH = N / 16;
+  const float4* p = reinterpret_cast<const float4*>(tw) + (DIR > 0 ? 16 * TH : 0) + t;
+#pragma unroll
+  for (int P = 0; P < twiddled_passes<N>(); P++) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const float4 v = p[(P * 8 + i) * TH];
+      T.w[P][2 * i] = make_float2(v.x, v.y);
+      T.w[P][2 * i + 1] = make_float2(v.z, v.w);
+    }
+  }
 }
 
 // Twiddle and DFT of a pass with radix RP whose sub-transforms have size NS
