@@ -219,10 +219,45 @@ __device__ __forceinline__ void line_sync() {
 
 // Runs the passes from sub-transform size NS on; v holds that pass's inputs.
 // Every thread of the block (WAVE = 0) or of the wave (WAVE = 1) must call it.
-template <int N, typename L, int DIR, int NS, int WAVE = 0, int P = 0>
+// DFT of size 16 (natural order in and out) of a line whose inputs 4..11 are
+// zero: the even outputs are the 8-point DFT of (x0..x3, x12..x15), the odd
+// ones the 8-point DFT of the same values times exp(DIR 2 pi i r / 16), r
+// their index (r = 12..15 contributing -exp(DIR 2 pi i (r-8) / 16)).
+template <int DIR>
+__device__ __forceinline__ void dft16_half(float2 (&v)[16]) {
+  float2 a[8], b[8];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    a[m] = v[m];
+    a[m + 4] = v[m + 12];
+  }
+  b[0] = v[0];
+  b[1] = rot16<1, DIR>(v[1]);
+  b[2] = rot16<2, DIR>(v[2]);
+  b[3] = rot16<3, DIR>(v[3]);
+  b[4] = rot16<4, DIR>(v[12]);
+  b[5] = rot16<5, DIR>(v[13]);
+  b[6] = rot16<6, DIR>(v[14]);
+  b[7] = rot16<7, DIR>(v[15]);
+#pragma unroll
+  for (int m = 4; m < 8; m++) b[m] = make_float2(-b[m].x, -b[m].y);
+  dft<8, DIR>(a);
+  dft<8, DIR>(b);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    v[2 * j] = a[j];
+    v[2 * j + 1] = b[j];
+  }
+}
+
+// HALF: the line's elements N/4 .. 3N/4-1 are zero (a zero-padded line
+// centred on element 0), so the inputs 4..11 of the first pass are zero and
+// are not read.
+template <int N, typename L, int DIR, int NS, int WAVE = 0, int P = 0, bool HALF = false>
 __device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {
   constexpr int RP = pass_radix<N, NS>();
-  pass_compute<N, RP, NS, DIR, P>(v, T);
+  if constexpr (HALF && NS == 1 && RP == 16) dft16_half<DIR>(v);
+  else pass_compute<N, RP, NS, DIR, P>(v, T);
   if constexpr (NS * RP < N) {
     pass_store<N, RP, NS, L>(v, lds, t);
     line_sync<WAVE>();
@@ -231,6 +266,13 @@ __device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, co
     line_sync<WAVE>();
     fft_from<N, L, DIR, NS * RP, WAVE, (NS > 1 ? P + 1 : 0)>(v, lds, t, T);
   }
+}
+
+// fft_regs of a zero-padded line: v[4..11] are zero and are not read.
+template <int N, typename L, int DIR>
+__device__ __forceinline__ void fft_regs_half(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {
+  static_assert(N >= 128, "the first pass is radix 16");
+  fft_from<N, L, DIR, 1, 0, 0, true>(v, lds, t, T);
 }
 
 // Registers in (input order) -> registers out (output order).  lds: scratch
@@ -254,6 +296,17 @@ __device__ __forceinline__ int out_index(int t, int i) {
   constexpr int RL = last_radix<N>();
   constexpr int T = N / 16;
   return t + (i / RL) * T + (i % RL) * (N / RL);
+}
+
+// True if output slot i of every thread holds only elements in [N/4, 3N/4):
+// the part of a line that cropping a zero-padded transform back to its
+// centred N/2 or fewer elements never keeps (its outputs need not be formed).
+template <int N>
+__host__ __device__ constexpr bool out_slot_cropped(int i) {
+  constexpr int RL = last_radix<N>();
+  constexpr int T = N / 16;
+  const int lo = (i / RL) * T + (i % RL) * (N / RL);
+  return lo >= N / 4 && lo + T <= 3 * N / 4;
 }
 
 // Element index held in input slot r of thread t.

@@ -59,9 +59,13 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
   const int h1 = a.N1 >> 1;
   FftTwiddles Tw;
   load_twiddles<NR, -1>(Tw, t, a.tw);
+  // the N1 data columns sit at y in [NR - N1/2, NR) and [0, N1 - N1/2), so
+  // with N1 <= NR/2 the inputs 4..11 of every thread are padding
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
+    v[r] = make_float2(0.f, 0.f);
+    if (r >= 4 && r < 12) continue;
     const int y = in_index<NR>(t, r);
     const int Y = (y + h1) & (NR - 1);
     float2 gv = make_float2(0.f, 0.f);
@@ -90,7 +94,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
     }
     v[r] = gv;
   }
-  fft_regs<NR, RowLay, -1>(v, lds + team * kRowLds<NR>, t, Tw);
+  fft_regs_half<NR, RowLay, -1>(v, lds + team * kRowLds<NR>, t, Tw);
   if (valid) {
     float2* o = a.out + (size_t)row * NR;
 #pragma unroll
@@ -122,6 +126,7 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 16; i++) {
+      if (out_slot_cropped<NR>(i)) continue;   // padding columns, as N1 <= NR/2
       const int y = out_index<NR>(t, i);
       const int Y = (y + h1) & (NR - 1);
       if (Y < a.N1) {
@@ -216,11 +221,14 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
+      // rows x in [NC/4, 3NC/4) (inputs 4..11) are padding, as N0 <= NC/2
+      v[r] = make_float2(0.f, 0.f);
+      if (r >= 4 && r < 12) continue;
       const int x = in_index<NC>(t, r);
       const int X = (x + h0) & (NC - 1);
-      v[r] = X < a.N0 ? a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c] : make_float2(0.f, 0.f);
+      if (X < a.N0) v[r] = a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c];
     }
-    fft_regs<NC, ColFftLay<C>, -1>(v, tile + c, t, T);
+    fft_regs_half<NC, ColFftLay<C>, -1>(v, tile + c, t, T);
     // the last exchange ended with a barrier: the column can be overwritten
 #pragma unroll
     for (int i = 0; i < 16; i++) tile[out_index<NC>(t, i) * LS + c] = v[i];
@@ -404,6 +412,7 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
     const int h0 = a.N0 >> 1;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
+      if (out_slot_cropped<NC>(i)) continue;   // padding rows, as N0 <= NC/2
       const int X = (out_index<NC>(t, i) + h0) & (NC - 1);
       if (X < a.N0) a.out[((size_t)s * a.N0 + X) * a.n1 + y0 + c] = v[i];
     }
@@ -524,6 +533,8 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
+    v[r] = make_float2(0.f, 0.f);
+    if (r >= 4 && r < 12) continue;   // padding, as N0 <= NR/2
     const int x = in_index<NR>(t, r);
     const int X = (x + h) & (NR - 1);
     float gv = 0.f;
@@ -531,7 +542,9 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
     v[r] = make_float2(gv, 0.f);
   }
   float2* L = lds + team * kRowLds<NR>;
-  fft_regs<NR, RowLay, -1>(v, L, t, a.tw);
+  FftTwiddles Tw;
+  load_twiddles<NR, -1>(Tw, t, a.tw);
+  fft_regs_half<NR, RowLay, -1>(v, L, t, Tw);
 #pragma unroll
   for (int i = 0; i < 16; i++) L[out_index<NR>(t, i)] = v[i];
   __syncthreads();
@@ -593,6 +606,7 @@ __global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
   const int h = a.N0 >> 1;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
+    if (out_slot_cropped<NR>(i)) continue;   // padding, as N0 <= NR/2
     const int y = out_index<NR>(t, i);
     const int X = (y + h) & (NR - 1);
     if (X < a.N0) a.out[(size_t)s * a.N0 + X] = v[i].x / a.ck1[X];
