@@ -80,6 +80,22 @@ def cpu_baseline(cfg, S_cpu, iters_cpu, total_iters, threads):
 COPY_BYTES = 1 << 30
 
 
+def iteration_roofline(plan, S, per_iter_ms):
+    """One ICON iteration against HBM with the reference's dataflow bytes per
+    slice, 17 G + 6 S + 8 I_c + 3 I_r (G = 8 n0 n1 grid, S = 8 M samples,
+    I_c = 8 N0 N1 complex image, I_r = 4 N0 N1 real image: three FFTs, pads,
+    spread, two interpolations, weights, residual and update as separate
+    passes).  Fusion moves fewer bytes, so the fraction can exceed 1."""
+    G = 8 * plan.n0 * plan.n1
+    Sb = 8 * plan.M
+    Ic = 8 * plan.N0 * plan.N1
+    Ir = 4 * plan.N0 * plan.N1
+    per_slice = 17 * G + 6 * Sb + 8 * Ic + 3 * Ir
+    ach = per_slice * S / (per_iter_ms * 1e-3) / 1e9
+    return dict(bytes_per_slice=per_slice, ms_per_iteration=round(per_iter_ms, 4), achieved=round(ach, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4))
+
+
 def probe(what, S):
     """Child-process mode for the counter passes: run the named ICON kernel on
     the workload's buffers (or a 1 GiB plain copy) a few times and exit."""
@@ -169,6 +185,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--col-block", type=int, default=0, help="grid columns per spread/interp tile (0: default)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-kb", action="store_true", help="skip the Kaiser-Bessel window variant")
     ap.add_argument("--probe", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.probe:
@@ -248,6 +265,35 @@ def main():
             roofline["traffic_detail"] = detail
             roofline["algo_bytes_per_launch"] = kb[dom] * S
         log("traffic:", roofline["traffic"])
+
+    # ---- Kaiser-Bessel variant: same workload, 6-tap window
+    kb6 = None
+    if not args.no_kb:
+        log("Kaiser-Bessel (width 6) plan: kernel timing and a plain solve")
+        pk = feilab_amd.Plan(cfg["N"], cfg["tilt"], max_slices=S, device=0, window="kb6")
+        kk = {}
+        for k in ("col_fwd", "col_adj"):
+            ms = pk.time_kernel(k, S, args.kernel_reps)
+            gbs = kb[k] * S / (ms * 1e-3) / 1e9
+            kk[k] = dict(ms=round(ms, 4), GBs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+        d_mk = DeviceArray((S, plan.N0, plan.N1), np.float32)
+        d_rk = DeviceArray((S, iters + 1), np.float64)
+        d_pk = DeviceArray((S, plan.N0), np.float32)
+        run_k = lambda: pk.reconstruct_dev(S, d_sino, d_mk, opts, d_resid=d_rk, d_reproj=d_pk)
+        dt_k = timed(run_k, "plain kb6")
+        R_k = d_rk.numpy()[:, -1]
+        _, frc_k = warmstart.reprojection_frc(d_pk.numpy(), cfg["sino"], iomit, band)
+        m_g, m_k = d_m.numpy(), d_mk.numpy()
+        kb6 = dict(window="Kaiser-Bessel, width 6, beta %.4f" % pk.info.b0, kernels=kk,
+                   plain_slices_per_s=round(S * args.steps / dt_k, 3),
+                   resid_max_over_gauss=round(float((R_k / R_plain).max()), 5),
+                   rel_l2_vs_gauss_max=float(np.max(np.linalg.norm((m_k - m_g).reshape(S, -1), axis=1)
+                                                    / np.linalg.norm(m_g.reshape(S, -1), axis=1))),
+                   frc=round(frc_k, 5))
+        log("kb6:", json.dumps(kb6))
+        for a in (d_mk, d_rk, d_pk):
+            a.free()
+        pk.close()
 
     # ---- learned leg
     learned = None
@@ -342,6 +388,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "iteration_roofline": iteration_roofline(plan, S, per_iter_ms),
+        "kb6_variant": kb6,
     }
     print(json.dumps(out), flush=True)
 
