@@ -209,8 +209,13 @@ def main():
     log("data ready in %.1f s" % (time.perf_counter() - t0))
 
     plan = feilab_amd.Plan(cfg["N"], cfg["tilt"], max_slices=S, device=0)
+    autotune = None
     if args.col_block:
         plan.set_col_block(args.col_block)
+    else:
+        c, ms = plan.autotune(S)
+        autotune = dict(col_block=c, ms_column_kernels_per_iteration={str(k): round(v, 4) for k, v in ms.items()})
+        log("autotuned column tile:", json.dumps(autotune))
     iomit = plan.info.iomit
     band = (1, plan.N0 // 4)
     d_sino = DeviceArray.from_host(cfg["sino"], np.float32)
@@ -383,6 +388,7 @@ def main():
             "plain_frc": round(frc_plain, 5),
             "learned": learned,
             "col_block": int(plan.info.col_block),
+            "autotune": autotune,
             "ms_per_iteration_kernels": round(per_iter_ms, 3),
         },
         "roofline": roofline,

@@ -758,6 +758,33 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   return ICON_OK;
 }
 
+int icon_plan_autotune(icon_plan* p, int S, int* col_block, double* ms_per_width) {
+  if (!p || !col_block) return fail(ICON_ERR_ARG, "icon_plan_autotune: null pointer");
+  const int widths[2] = {8, 16};
+  double best = 0.0;
+  int best_c = 0;
+  for (int i = 0; i < 2; i++) {
+    const int C = widths[i];
+    if (ms_per_width) ms_per_width[i] = 0.0;
+    if (!col_config_ok(p->g.n0, C, p->g.K) || p->g.n1 % C != 0) continue;
+    int rc = icon_plan_set_col_block(p, C);
+    double tf = 0.0, ta = 0.0;
+    if (!rc) rc = icon_time_kernel(p, ICON_K_COL_FWD, S, 3, &tf);
+    if (!rc) rc = icon_time_kernel(p, ICON_K_COL_ADJ, S, 3, &ta);
+    if (rc) return rc;
+    const double t = 2.0 * tf + ta;
+    if (ms_per_width) ms_per_width[i] = t;
+    if (best_c == 0 || t < best) { best = t; best_c = C; }
+  }
+  if (best_c == 0) return fail(ICON_ERR_ARG, "icon_plan_autotune: no supported column tile");
+  if (p->bins.C != best_c) {
+    const int rc = icon_plan_set_col_block(p, best_c);
+    if (rc) return rc;
+  }
+  *col_block = best_c;
+  return ICON_OK;
+}
+
 int icon_copy_probe(void* d_dst, const void* d_src, size_t bytes, int reps, double* avg_ms) {
   if (!d_dst || !d_src || !avg_ms || reps < 1 || bytes % 16 != 0)
     return fail(ICON_ERR_ARG, "icon_copy_probe: bad argument (bytes must be a multiple of 16)");

@@ -87,6 +87,7 @@ def lib():
         "icon_plan_destroy": [vp],
         "icon_plan_get_info": [vp, ctypes.POINTER(PlanInfo)],
         "icon_plan_set_col_block": [vp, ci],
+        "icon_plan_autotune": [vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_double)],
         "icon_nufft_forward_real": [vp, ci, vp, vp, vp],
         "icon_nufft_forward_complex": [vp, ci, vp, vp, vp],
         "icon_nufft_adjoint": [vp, ci, vp, vp, vp],
@@ -237,6 +238,14 @@ class Plan:
     def set_col_block(self, c):
         _check(lib().icon_plan_set_col_block(self.h, int(c)), "icon_plan_set_col_block")
         self.info.col_block = int(c)
+
+    def autotune(self, S):
+        """Picks the fastest column tile for S slices; returns (C, {C: ms per iteration's column kernels})."""
+        c = ctypes.c_int(0)
+        ms = (ctypes.c_double * 2)()
+        _check(lib().icon_plan_autotune(self.h, int(S), ctypes.byref(c), ms), "icon_plan_autotune")
+        self.info.col_block = c.value
+        return c.value, {w: ms[i] for i, w in enumerate((8, 16)) if ms[i] > 0}
 
     # -- batched operators (device in / device out) -------------------------
     def forward_real_dev(self, S, d_img, d_f, stream=None):

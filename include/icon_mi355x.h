@@ -121,6 +121,14 @@ int icon_plan_get_info(const icon_plan* plan, icon_plan_info* info);
 /* Tune the spread/interp column tile (8 or 16 grid columns); rebuilds the
  * bin lists.  Optional. */
 int icon_plan_set_col_block(icon_plan* plan, int col_block);
+/* Autotuner of the column tile (the spread/interpolation tile and bin width):
+ * times one iteration's column kernels (2 interpolations + 1 spread, each
+ * after its predecessor) over S slices for every tile width the plan's grid
+ * and window support, keeps the fastest and reports it in *col_block and, if
+ * ms_per_width is not NULL, the measured ms in ms_per_width[0] (C = 8) and
+ * [1] (C = 16, or 0 when unsupported).  Overwrites the solve buffers, like
+ * icon_time_kernel.  No counterpart in the reference. */
+int icon_plan_autotune(icon_plan* plan, int S, int* col_block, double* ms_per_width);
 
 /* ---- operators on S slices (device pointers, batched) ------------------ */
 /* Forward NUFFT of S real images [S][N0][N1] to S node vectors [S][M]
