@@ -268,6 +268,7 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   RowAdjArgs ra{};
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 1; ra.in = p->d_tmp; ra.ck = p->d_ck;
   ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart;
+  ra.fwd_out = p->d_tmp;   // the row pass of A rhat fused in
   auto phase = [&](int it) { return (o.pre <= it && it < iters - o.post) ? 2 : 1; };
 
   for (int it = 0; it < iters; it++) {
@@ -280,13 +281,10 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
     cm.resid_index = it;
     cm.frep = nullptr;
     if ((rc = check_launch(launch_combine(cm, st), "combine_m"))) return rc;
-    // rhat = A^H fw - A^H W b, row sums of |rhat|^2
+    // rhat = A^H fw - A^H W b, row sums of |rhat|^2, and the row pass of A rhat
     if ((rc = check_launch(launch_col(g.n0, p->bins.C, true, ca, st), "col_adj"))) return rc;
     if ((rc = check_launch(launch_row_adj(g.n1, ra, st), "row_adj"))) return rc;
     // alpha = |rhat|^2 / sum w |A rhat|^2
-    rf.cplx = 1;
-    rf.update = 0;
-    if ((rc = check_launch(launch_row_fwd(g.n1, rf, st), "row_fwd_c"))) return rc;
     if ((rc = check_launch(launch_col(g.n0, p->bins.C, false, ca, st), "col_fwd_c"))) return rc;
     cr.alpha_index = it;
     if ((rc = check_launch(launch_combine(cr, st), "combine_r"))) return rc;
@@ -709,6 +707,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   RowAdjArgs ra{};
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 1; ra.in = p->d_tmp; ra.ck = p->d_ck;
   ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart;
+  ra.fwd_out = p->d_tmp;
   SinoArgs sa{};
   sa.K = p->g.K;
   sa.rows = S * g.nz; sa.N0 = g.N0; sa.sino = reinterpret_cast<const float*>(p->d_b); sa.ck1 = p->d_ck1;
@@ -732,7 +731,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
       case ICON_K_COMBINE: return launch_col(g.n0, p->bins.C, false, ca, st);
       case ICON_K_COL_ADJ: return launch_combine(cm, st);
       case ICON_K_ROW_ADJ: return launch_col(g.n0, p->bins.C, true, ca, st);
-      case ICON_K_ROW_FWD: return launch_row_adj(g.n1, ra, st);
+      case ICON_K_ROW_FWD: return launch_combine(cm, st);
       default: return hipSuccess;
     }
   };

@@ -34,6 +34,7 @@ struct RowAdjArgs {
   const float2* ahwb;  // mode 1
   float2* rhat;        // mode 1 output
   double* rowpart;     // mode 1: sum |rhat|^2 per row
+  float2* fwd_out;     // mode 1, or null: y-FFT of rhat/ck padded to NR, per row [S][N0][NR]
 };
 
 struct ColArgs {

@@ -103,8 +103,12 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
 }
 
 
-template <int NR, int RPB>
-__global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
+// FWD: also the first row pass of A rhat (what k_row_fwd does for a complex
+// image): rhat / ck padded to NR and y-transformed, written to a.fwd_out.
+// The inverse's output slot i and the forward's input slot r of a thread hold
+// the same element t + k T, so the hand-over is a fixed register permutation.
+template <int NR, int RPB, bool FWD>
+__global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
   constexpr int T = NR / 16;
   __shared__ float2 lds[RPB * kRowLds<NR>];
   __shared__ double red[RPB * T];
@@ -123,6 +127,9 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
   fft_regs<NR, RowLay, 1>(v, lds + team * kRowLds<NR>, t, Tw);
   const int h1 = a.N1 >> 1;
   double acc = 0.0;
+  float2 u[16];   // FWD: forward input, slot r holds element t + r T
+#pragma unroll
+  for (int r = 0; r < 16; r++) u[r] = make_float2(0.f, 0.f);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -140,6 +147,10 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
           const float2 r = make_float2(f.x - w.x, f.y - w.y);
           a.rhat[idx] = r;
           acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+          if constexpr (FWD) {
+            constexpr int RL = last_radix<NR>();
+            u[i / RL + (i % RL) * (16 / RL)] = make_float2(r.x / c, r.y / c);
+          }
         }
       }
     }
@@ -151,6 +162,16 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_adj(RowAdjArgs a) {
       double sum = 0.0;
       for (int i = 0; i < T; i++) sum += red[team * T + i];
       a.rowpart[row] = sum;
+    }
+  }
+  if constexpr (FWD) {
+    // the inverse's last LDS exchange ended with a barrier
+    load_twiddles<NR, -1>(Tw, t, a.tw);
+    fft_regs_half<NR, RowLay, -1>(u, lds + team * kRowLds<NR>, t, Tw);
+    if (valid) {
+      float2* o = a.fwd_out + (size_t)row * NR;
+#pragma unroll
+      for (int i = 0; i < 16; i++) o[out_index<NR>(t, i)] = u[i];
     }
   }
 }
@@ -717,8 +738,12 @@ hipError_t launch_row_adj(int NR, const RowAdjArgs& a, hipStream_t st) {
 #define CASE(N)                                                                                     \
   case N: {                                                                                         \
     constexpr int RPB = rows_per_block<N>();                                                        \
-    hipLaunchKernelGGL((k_row_adj<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
-                       st, a);                                                                      \
+    if (h.fwd_out)                                                                                  \
+      hipLaunchKernelGGL((k_row_adj<N, RPB, true>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+                         st, a);                                                                    \
+    else                                                                                            \
+      hipLaunchKernelGGL((k_row_adj<N, RPB, false>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+                         st, a);                                                                    \
     break;                                                                                          \
   }
     ROW_CASES(CASE)

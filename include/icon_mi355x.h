@@ -194,7 +194,9 @@ typedef enum {
   ICON_K_COL_FWD = 1,   /* x-FFT of the grid tile + interpolation to the nodes */
   ICON_K_COMBINE = 2,   /* node sums, weights, residual (FP64)                 */
   ICON_K_COL_ADJ = 3,   /* spread of the nodes into the grid tile + x-IFFT     */
-  ICON_K_ROW_ADJ = 4,   /* y-IFFT + crop + deapodise + gradient + |r|^2 (FP64) */
+  ICON_K_ROW_ADJ = 4,   /* y-IFFT + crop + deapodise + gradient + |r|^2 (FP64)
+                           + deapodise, pad and y-FFT of the gradient (the row
+                           pass of A rhat, fused)                              */
   ICON_K_SINO = 5       /* 1D transform of the projection rows                 */
 } icon_kernel_id;
 
