@@ -77,58 +77,100 @@ __device__ __forceinline__ void store1(float* p, float v) { *p = v; }
 // reads with ds_read_b128 fall in distinct banks); wave w then multiplies
 // tile rows 4w..4w+3 (4 MFMA row tiles of 16 pixels) by the COB/16 channel
 // tiles for every tap, reading both operands from LDS.
+//
+// The grid is persistent: a block walks the work items (tile, chunk) of its
+// output-channel block, tiles blockIdx.x, blockIdx.x + gridDim.x, ..., and
+// loads the next item's input and weights into registers while the MFMAs of
+// the current one run; the weights are loaded once when the input has a
+// single chunk.
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
 constexpr int kLdsRow = 40;   // bf16 per staged pixel / weight row (32 + 8 pad)
+constexpr int kInQ = kHalo * kHalo * 4;            // 16-byte pieces of a staged input tile
+constexpr int kInPer = (kInQ + 255) / 256;         // per thread
 
 template <typename TX, typename TY, int COB>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs<TX, TY> a) {
+__global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY> a) {
   constexpr int CT = COB / 16;
+  constexpr int kWPer = (9 * COB * 4 + 255) / 256;   // weight pieces per thread (9 taps)
   __shared__ __align__(16) uint16_t s_in[kHalo * kHalo * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
   const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
-  int bid = blockIdx.x;
-  const int tx = bid % tiles_x;
-  bid /= tiles_x;
-  const int ty = bid % tiles_y;
-  const int n = bid / tiles_y;
-  const int x0 = tx * kTile, y0 = ty * kTile;
+  const int ntiles = a.N * tiles_x * tiles_y;
+  const int nck = (a.Cin + 31) / 32;
   const int co0 = blockIdx.y * COB;
   const int taps = a.taps;
+  const int wq = taps * COB * 4;
+  int tt = blockIdx.x;
+  if (tt >= ntiles) return;
+  uint4 pin[kInPer];
+  // loads the input tile + halo of work item (tile, chunk) into registers
+  auto fetch = [&](int tile, int ck) {
+    const int tx = tile % tiles_x, rest = tile / tiles_x;
+    const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
+    const int ci0 = 32 * ck;
+#pragma unroll
+    for (int i = 0; i < kInPer; i++) {
+      const int q = tid + 256 * i;
+      const int c = q & 3, pix = q >> 2;
+      const int yy = y0 + pix / kHalo - 1, xx = x0 + pix % kHalo - 1;
+      const int ci = ci0 + 8 * c;
+      const bool ok = q < kInQ && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ci < a.Cin;
+      const TX* src = a.x + (((long)n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.ldx + a.cx0 + (ok ? ci : 0);
+      V8 v;
+      v.v = load8<TX>(src, ok);
+      pin[i] = v.u;
+    }
+  };
+  // stores the prefetched input and, when they change, the chunk's weights
+  // (read straight from L2: every block of the launch reads the same ones)
+  auto put = [&](int ck, bool with_w) {
+#pragma unroll
+    for (int i = 0; i < kInPer; i++) {
+      const int q = tid + 256 * i;
+      if (q < kInQ) *reinterpret_cast<uint4*>(&s_in[(q >> 2) * kLdsRow + 8 * (q & 3)]) = pin[i];
+    }
+    if (with_w) {
+      const int ci0 = 32 * ck;
+#pragma unroll
+      for (int i = 0; i < kWPer; i++) {
+        const int q = tid + 256 * i;
+        const int c = q & 3, r = q >> 2;
+        const int co = co0 + r % COB, t = r / COB, ci = ci0 + 8 * c;
+        const bool ok = q < wq && co < a.Cout && ci < a.Cin;
+        if (q < wq)
+          *reinterpret_cast<uint4*>(&s_w[r * kLdsRow + 8 * c]) =
+              ok ? *reinterpret_cast<const uint4*>(a.w + ((long)t * a.Cout + co) * a.Cin + ci) : make_uint4(0, 0, 0, 0);
+      }
+    }
+  };
   f4 acc[4][CT];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < CT; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
-    // input tile + halo: kHalo^2 pixels x 4 chunks of 8 channels
-    for (int q = tid; q < kHalo * kHalo * 4; q += 256) {
-      const int c = q & 3, pix = q >> 2;
-      const int yy = y0 + pix / kHalo - 1, xx = x0 + pix % kHalo - 1;
-      const int ci = ci0 + 8 * c;
-      const bool ok = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ci < a.Cin;
-      const TX* src = a.x + (((long)n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.ldx + a.cx0 + (ok ? ci : 0);
-      V8 v;
-      v.v = load8<TX>(src, ok);
-      *reinterpret_cast<uint4*>(&s_in[pix * kLdsRow + 8 * c]) = v.u;
-    }
-    // weights of every tap: taps x COB rows x 4 chunks
-    for (int q = tid; q < taps * COB * 4; q += 256) {
-      const int c = q & 3, r = q >> 2;
-      const int co = co0 + r % COB, t = r / COB, ci = ci0 + 8 * c;
-      const bool ok = co < a.Cout && ci < a.Cin;
-      const uint4 v = ok ? *reinterpret_cast<const uint4*>(a.w + ((long)t * a.Cout + co) * a.Cin + ci)
-                         : make_uint4(0, 0, 0, 0);
-      *reinterpret_cast<uint4*>(&s_w[r * kLdsRow + 8 * c]) = v;
-    }
+  int ck = 0;
+  bool w_in = true;   // the weights in LDS are not this chunk's
+  fetch(tt, 0);
+  for (;;) {
+    put(ck, w_in);
     __syncthreads();
+    // the next work item; its loads are in flight during this item's MFMAs
+    int tt2 = tt, ck2 = ck + 1;
+    if (ck2 == nck) {
+      ck2 = 0;
+      tt2 = tt + gridDim.x;
+    }
+    const bool more = tt2 < ntiles;
+    w_in = nck > 1;
+    if (more) fetch(tt2, ck2);
     if (taps == 9) {
       // for a column offset dx the A operand of tile row i at row offset dy is
       // halo row 4w + i + dy: six halo rows serve all three dy
-#pragma unroll
+#pragma unroll 1
       for (int dx = 0; dx < 3; dx++) {
         bf8 A[6];
 #pragma unroll
@@ -171,31 +213,42 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs<TX, TY> a) {
         for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v.v, B[j], acc[i][j], 0, 0, 0);
       }
     }
-    __syncthreads();
-  }
-  // epilogue: D[pixel 4g + r of row tile i][channel r16 of channel tile j]
+    if (ck == nck - 1) {
+      // epilogue: D[pixel 4g + r of row tile i][channel r16 of channel tile j]
+      const int tx = tt % tiles_x, rest = tt / tiles_x;
+      const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
 #pragma unroll
-  for (int j = 0; j < CT; j++) {
-    const int co = co0 + 16 * j + r16;
-    if (co >= a.Cout) continue;
-    const float bias = a.bias ? a.bias[co] : 0.f;
+      for (int j = 0; j < CT; j++) {
+        const int co = co0 + 16 * j + r16;
+        if (co >= a.Cout) continue;
+        const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int y = y0 + 4 * wave + i;
-      if (y >= a.H) continue;
+        for (int i = 0; i < 4; i++) {
+          const int y = y0 + 4 * wave + i;
+          if (y >= a.H) continue;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int x = x0 + 4 * g + r;
-        if (x >= a.W) continue;
-        const long p = ((long)n * a.H + y) * a.W + x;
-        float v = acc[i][j][r] + bias;
-        if (a.relu) v = fmaxf(v, 0.f);
-        if (a.mask && !(bf2f(a.mask[p * a.ldm + a.cm0 + co]) > 0.f)) v = 0.f;
-        TY* dst = a.y + p * a.ldy + a.cy0 + co;
-        if (a.accumulate) v += load1(dst);
-        store1(dst, v);
+          for (int r = 0; r < 4; r++) {
+            const int x = x0 + 4 * g + r;
+            if (x >= a.W) continue;
+            const long p = ((long)n * a.H + y) * a.W + x;
+            float v = acc[i][j][r] + bias;
+            if (a.relu) v = fmaxf(v, 0.f);
+            if (a.mask && !(bf2f(a.mask[p * a.ldm + a.cm0 + co]) > 0.f)) v = 0.f;
+            TY* dst = a.y + p * a.ldy + a.cy0 + co;
+            if (a.accumulate) v += load1(dst);
+            store1(dst, v);
+          }
+        }
       }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < CT; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     }
+    __syncthreads();   // the LDS operands are consumed before the next put
+    if (!more) break;
+    tt = tt2;
+    ck = ck2;
   }
 }
 
@@ -564,10 +617,22 @@ template <typename TX, typename TY>
 hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
   if (a.taps != 9 && a.taps != 1) return hipErrorInvalidValue;
   const long tiles = (long)a.N * ((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
+  // persistent grid: the blocks that fit the device at once (LDS-bound:
+  // 3 per CU at COB 32, 2 at COB 64), over all output-channel blocks
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+  }
   if (a.Cout <= 32) {
-    hipLaunchKernelGGL((k_conv<TX, TY, 32>), dim3((unsigned)tiles, (unsigned)((a.Cout + 31) / 32)), dim3(256), 0, st, a);
+    const int cb = (a.Cout + 31) / 32;
+    const long gx = std::max<long>(1, std::min<long>(tiles, 3L * cus / cb));
+    hipLaunchKernelGGL((k_conv<TX, TY, 32>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
   } else {
-    hipLaunchKernelGGL((k_conv<TX, TY, 64>), dim3((unsigned)tiles, (unsigned)((a.Cout + 63) / 64)), dim3(256), 0, st, a);
+    const int cb = (a.Cout + 63) / 64;
+    const long gx = std::max<long>(1, std::min<long>(tiles, 2L * cus / cb));
+    hipLaunchKernelGGL((k_conv<TX, TY, 64>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
   }
   return hipGetLastError();
 }
