@@ -136,9 +136,13 @@ int refresh_bf16(icon_unet* n, hipStream_t st) {
   return 0;
 }
 
+// up (optional): the input's first Cup channels, read from this half-size
+// tensor (nearest-neighbour 2x upsampling) instead of x; pool (optional):
+// the output's 2x2 average, written at half size
 template <typename TX, typename TY>
 int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int ldy, int cy0, int B, int H, int W,
-         bool relu, const uint16_t* mask, int ldm, int cm0, bool flipped, bool accumulate, hipStream_t st) {
+         bool relu, const uint16_t* mask, int ldm, int cm0, bool flipped, bool accumulate, hipStream_t st,
+         const T16* up = nullptr, const T16* pool = nullptr) {
   ConvArgs<TX, TY> a{};
   a.x = x; a.ldx = ldx; a.cx0 = cx0;
   a.w = flipped ? c.wt16 : c.w16;
@@ -150,6 +154,8 @@ int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int 
   a.Cout = flipped ? c.cin : c.cout;
   a.taps = c.taps;
   a.relu = relu; a.accumulate = accumulate;
+  if (up) { a.up = up->p; a.ldu = up->C; a.cu0 = 0; a.Cup = up->C; }
+  if (pool) { a.pool = pool->p; a.ldp = pool->C; a.cp0 = 0; }
   UCHK(launch_conv(a, st));
   return 0;
 }
@@ -167,11 +173,6 @@ int wgrad(icon_unet* n, const Conv& c, const uint16_t* x, int ldx, int cx0, cons
   return 0;
 }
 
-int pool(const T16& in, int ci0, int C, const T16& out, int B, int H, int W, hipStream_t st) {
-  PoolArgs a{in.p, in.C, ci0, out.p, out.C, 0, B, H, W, C};
-  UCHK(launch_pool_fwd(a, st));
-  return 0;
-}
 int up(const T16& in, const T16& out, int co0, int B, int h, int w, hipStream_t st) {
   PoolArgs a{in.p, in.C, 0, out.p, out.C, co0, B, h, w, in.C};
   UCHK(launch_up_fwd(a, st));
@@ -190,24 +191,32 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   if (ce != hipSuccess) return ufail(2, "input conversion: %s", hipGetErrorString(ce));
   UCHK(launch_conv_first(n->x8.p, 8, cv[E1A].w16, cv[E1A].cin, n->params + cv[E1A].off_b, n->a1.p, 32, B, H, W, 32,
                          st));
-  R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, 96, 64, B, H, W, true, nullptr, 0, 0, false, false, st));
-  R(pool(n->cat1, 64, 32, n->p1, B, H, W, st));
+  // the encoder convolutions also write their 2x2 average pool; without
+  // training (no weight gradients need the concatenated decoder inputs) the
+  // decoder convolutions read the upsampled half of their input straight
+  // from the half-size tensor
+  const bool fuse_up = !n->train;
+  R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, 96, 64, B, H, W, true, nullptr, 0, 0, false, false, st, nullptr,
+         &n->p1));
   R(conv(n, cv[E2A], n->p1.p, 32, 0, n->a2.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
-  R(conv(n, cv[E2B], n->a2.p, 64, 0, n->cat2.p, 192, 128, B, H2, W2, true, nullptr, 0, 0, false, false, st));
-  R(pool(n->cat2, 128, 64, n->p2, B, H2, W2, st));
+  R(conv(n, cv[E2B], n->a2.p, 64, 0, n->cat2.p, 192, 128, B, H2, W2, true, nullptr, 0, 0, false, false, st, nullptr,
+         &n->p2));
   R(conv(n, cv[E3A], n->p2.p, 64, 0, n->a3.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st));
-  R(conv(n, cv[E3B], n->a3.p, 128, 0, n->cat3.p, 384, 256, B, H4, W4, true, nullptr, 0, 0, false, false, st));
-  R(pool(n->cat3, 256, 128, n->p3, B, H4, W4, st));
+  R(conv(n, cv[E3B], n->a3.p, 128, 0, n->cat3.p, 384, 256, B, H4, W4, true, nullptr, 0, 0, false, false, st, nullptr,
+         &n->p3));
   R(conv(n, cv[BA], n->p3.p, 128, 0, n->ab.p, 256, 0, B, H8, W8, true, nullptr, 0, 0, false, false, st));
   R(conv(n, cv[BB], n->ab.p, 256, 0, n->bb.p, 256, 0, B, H8, W8, true, nullptr, 0, 0, false, false, st));
-  R(up(n->bb, n->cat3, 0, B, H8, W8, st));
-  R(conv(n, cv[D3A], n->cat3.p, 384, 0, n->d3a.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st));
+  if (!fuse_up) R(up(n->bb, n->cat3, 0, B, H8, W8, st));
+  R(conv(n, cv[D3A], n->cat3.p, 384, 0, n->d3a.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st,
+         fuse_up ? &n->bb : nullptr));
   R(conv(n, cv[D3B], n->d3a.p, 128, 0, n->d3b.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st));
-  R(up(n->d3b, n->cat2, 0, B, H4, W4, st));
-  R(conv(n, cv[D2A], n->cat2.p, 192, 0, n->d2a.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
+  if (!fuse_up) R(up(n->d3b, n->cat2, 0, B, H4, W4, st));
+  R(conv(n, cv[D2A], n->cat2.p, 192, 0, n->d2a.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st,
+         fuse_up ? &n->d3b : nullptr));
   R(conv(n, cv[D2B], n->d2a.p, 64, 0, n->d2b.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
-  R(up(n->d2b, n->cat1, 0, B, H2, W2, st));
-  R(conv(n, cv[D1A], n->cat1.p, 96, 0, n->d1a.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
+  if (!fuse_up) R(up(n->d2b, n->cat1, 0, B, H2, W2, st));
+  R(conv(n, cv[D1A], n->cat1.p, 96, 0, n->d1a.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st,
+         fuse_up ? &n->d2b : nullptr));
   R(conv(n, cv[D1B], n->d1a.p, 32, 0, n->d1b.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
   if (launch_head_fwd(n->d1b.p, 32, n->params + n->head_w, n->params + n->head_b, 32, P, d_out, st) != hipSuccess)
     return ufail(2, "head");

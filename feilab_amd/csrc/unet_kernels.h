@@ -22,6 +22,14 @@ struct ConvArgs {
   int ldm, cm0;
   int N, H, W, Cin, Cout, taps;
   int relu, accumulate;
+  // optional: input channels [0, Cup) read from `up` (H/2 x W/2 pixels,
+  // nearest-neighbour 2x upsampling) instead of x
+  const uint16_t* up;
+  int ldu, cu0, Cup;
+  // optional (BF16 output): 2x2 average pool of the output, written to pool
+  // (H/2 x W/2 pixels)
+  uint16_t* pool;
+  int ldp, cp0;
 };
 
 // dw[tap][co][ci] += sum over pixels of dy[p][co] * x[p + offset(tap)][ci]
@@ -58,7 +66,6 @@ template <typename TX, typename TY>
 hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st);
 hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st);
 hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db, hipStream_t st);
-hipError_t launch_pool_fwd(const PoolArgs& a, hipStream_t st);
 hipError_t launch_pool_bwd(const PoolGradArgs& a, hipStream_t st);
 hipError_t launch_up_fwd(const PoolArgs& a, hipStream_t st);
 hipError_t launch_up_bwd(const PoolGradArgs& a, hipStream_t st);

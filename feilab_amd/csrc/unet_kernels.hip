@@ -119,8 +119,14 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
       const int yy = y0 + pix / kHalo - 1, xx = x0 + pix % kHalo - 1;
       const int ci = ci0 + 8 * c;
       const bool ok = q < kInQ && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ci < a.Cin;
-      const TX* src = a.x + (((long)n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.ldx + a.cx0 + (ok ? ci : 0);
       V8 v;
+      const TX* src = a.x + (((long)n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.ldx + a.cx0 + (ok ? ci : 0);
+      if constexpr (sizeof(TX) == 2) {
+        // upsampled channels: the same load from the half-size tensor
+        if (ci < a.Cup)
+          src = a.up + (((long)n * (a.H >> 1) + (ok ? yy >> 1 : 0)) * (a.W >> 1) + (ok ? xx >> 1 : 0)) * a.ldu + a.cu0 +
+                (ok ? ci : 0);
+      }
       v.v = load8<TX>(src, ok);
       pin[i] = v.u;
     }
@@ -237,6 +243,32 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
             TY* dst = a.y + p * a.ldy + a.cy0 + co;
             if (a.accumulate) v += load1(dst);
             store1(dst, v);
+          }
+        }
+        if constexpr (sizeof(TY) == 2) {
+          if (a.pool) {
+            // 2x2 average of the stored BF16 values, summed in the order
+            // (y, x), (y, x+1), (y+1, x), (y+1, x+1): rows 4w+i and 4w+i+1,
+            // columns 4g+r and 4g+r+1 are all in this lane
+#pragma unroll
+            for (int i = 0; i < 4; i += 2) {
+              const int y = y0 + 4 * wave + i;
+              if (y + 1 >= a.H) continue;
+#pragma unroll
+              for (int r = 0; r < 4; r += 2) {
+                const int x = x0 + 4 * g + r;
+                if (x + 1 >= a.W) continue;
+                float q4[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                  float v = acc[i + (k >> 1)][j][r + (k & 1)] + bias;
+                  if (a.relu) v = fmaxf(v, 0.f);
+                  q4[k] = bf2f(f2bf(v));
+                }
+                const long pp = ((long)n * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+                a.pool[pp * a.ldp + a.cp0 + co] = f2bf(0.25f * (q4[0] + q4[1] + q4[2] + q4[3]));
+              }
+            }
           }
         }
       }
@@ -360,28 +392,6 @@ __global__ __launch_bounds__(256) void k_bias_grad(const float* dy, int ld, int 
 
 // 2x2 average pool (bf16 NHWC, channel sub-ranges); 8 channels per thread
 // (C, strides and channel offsets are multiples of 8).
-__global__ void k_pool_fwd(PoolArgs a) {
-  const int C8 = a.C / 8;
-  const long n = (long)a.N * (a.H / 2) * (a.W / 2) * C8;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int c = 8 * (int)(i % C8);
-    const long q = i / C8;
-    const int x = (int)(q % (a.W / 2)), y = (int)((q / (a.W / 2)) % (a.H / 2));
-    const long nn = q / ((long)(a.W / 2) * (a.H / 2));
-    const long b = ((nn * a.H + 2 * y) * a.W + 2 * x) * a.ldi + a.ci0 + c;
-    V8 v0, v1, v2, v3, o;
-    v0.u = *reinterpret_cast<const uint4*>(a.in + b);
-    v1.u = *reinterpret_cast<const uint4*>(a.in + b + a.ldi);
-    v2.u = *reinterpret_cast<const uint4*>(a.in + b + (long)a.W * a.ldi);
-    v3.u = *reinterpret_cast<const uint4*>(a.in + b + (long)a.W * a.ldi + a.ldi);
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-      o.h[k] = f2bf(0.25f * (bf2f(v0.h[k]) + bf2f(v1.h[k]) + bf2f(v2.h[k]) + bf2f(v3.h[k])));
-    *reinterpret_cast<uint4*>(a.out + q * a.ldo + a.co0 + c) = o.u;
-  }
-}
-
-// gradient of the pool: dIn[2y+u][2x+v] (+)= dOut[y][x] / 4  (fp32)
 __global__ void k_pool_bwd(PoolGradArgs a) {
   const long n = (long)a.N * a.H * a.W * a.C;   // H, W of the input
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -650,10 +660,6 @@ hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, floa
   return hipGetLastError();
 }
 
-hipError_t launch_pool_fwd(const PoolArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C / 32)), dim3(256), 0, st, a);
-  return hipGetLastError();
-}
 hipError_t launch_pool_bwd(const PoolGradArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((long)a.N * a.H * a.W * a.C)), dim3(256), 0, st, a);
   return hipGetLastError();

@@ -147,3 +147,18 @@ def test_unet_warm_start_is_scaled_residual(icon):
     for i in range(5):
         assert rel(out[i], ref[i]) < 2e-3, i   # host and device round m / s to BF16 separately
     net.close()
+
+
+def test_unet_inference_net_equals_training_net(icon):
+    """The inference net reads its decoders' upsampled inputs straight from
+    the half-size tensors; the training net (checked against the oracle above)
+    materialises them.  Same parameters, same input: identical output."""
+    from feilab_amd.unet import UNet
+    _print("unet inference vs training path")
+    tr, p = _net_and_params(64, 48, 3, seed=21)
+    inf = UNet(64, 48, max_batch=3)
+    inf.set_params(p)
+    x = np.random.default_rng(5).standard_normal((3, 64, 48)).astype(np.float32)
+    np.testing.assert_array_equal(inf.forward(x), tr.forward(x))
+    inf.close()
+    tr.close()
