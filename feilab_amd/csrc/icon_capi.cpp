@@ -6,6 +6,7 @@
 //   reprojection          MainCode.cu:142, reProject_NFFT.cu:81-98
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -739,21 +740,21 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   hipError_t e = launch();  // warm
-  double total = 0.0;
+  std::vector<float> t(reps, 0.f);
   for (int r = 0; r < reps && e == hipSuccess; r++) {
     e = before();
     if (e == hipSuccess) e = hipEventRecord(e0, st);
     if (e == hipSuccess) e = launch();
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    total += ms;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t[r], e0, e1);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "icon_time_kernel: %s", hipGetErrorString(e));
-  *avg_ms = total / reps;
+  // median launch duration: robust to a launch slowed by other work on the device
+  std::sort(t.begin(), t.end());
+  *avg_ms = reps % 2 ? t[reps / 2] : 0.5 * ((double)t[reps / 2 - 1] + t[reps / 2]);
   return ICON_OK;
 }
 

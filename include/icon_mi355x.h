@@ -203,7 +203,7 @@ typedef enum {
 /* Times `reps` launches of one kernel of the iteration over S slices, each
  * right after the kernel that precedes it in an iteration (untimed), with
  * HIP events around the timed launch on the plan's stream, on the buffers of
- * the plan's last solve; *avg_ms = average launch duration.  Not part of the
+ * the plan's last solve; *avg_ms = median launch duration.  Not part of the
  * reference; measurement only. */
 int icon_time_kernel(icon_plan* plan, int kernel, int S, int reps, double* avg_ms);
 
