@@ -33,6 +33,13 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
+// Barrier of one wavefront for data it exchanges through LDS.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 union V8 {
   uint4 u;
   uint16_t h[8];
@@ -95,11 +102,17 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
   constexpr int kWPer = (9 * COB * 4 + 255) / 256;   // weight pieces per thread (9 taps)
   __shared__ __align__(16) uint16_t s_in[kHalo * kHalo * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
+  // per wave: one row tile of BF16 output (16 pixels x COB channels), the
+  // transpose that turns the MFMA layout into 16-byte stores of 8 channels
+  __shared__ __align__(16) uint16_t s_out[4 * kTile * COB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
   const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
   const int ntiles = a.N * tiles_x * tiles_y;
+  // plain BF16 output (no mask, no accumulation), 8-channel aligned
+  const bool vec_out = sizeof(TY) == 2 && !a.mask && !a.accumulate && a.Cout % 8 == 0 && a.ldy % 8 == 0 &&
+                       a.cy0 % 8 == 0;
   const int nck = (a.Cin + 31) / 32;
   const int co0 = blockIdx.y * COB;
   const int taps = a.taps;
@@ -223,6 +236,41 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
       // epilogue: D[pixel 4g + r of row tile i][channel r16 of channel tile j]
       const int tx = tt % tiles_x, rest = tt / tiles_x;
       const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
+      if constexpr (sizeof(TY) == 2) {
+        if (vec_out) {
+          uint16_t* ep = s_out + wave * kTile * COB;
+          float bias[CT];
+#pragma unroll
+          for (int j = 0; j < CT; j++) {
+            const int co = co0 + 16 * j + r16;
+            bias[j] = a.bias && co < a.Cout ? a.bias[co] : 0.f;
+          }
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int y = y0 + 4 * wave + i;
+#pragma unroll
+            for (int j = 0; j < CT; j++)
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                float v = acc[i][j][r] + bias[j];
+                if (a.relu) v = fmaxf(v, 0.f);
+                ep[(4 * g + r) * COB + 16 * j + r16] = f2bf(v);
+              }
+            wave_sync();
+            // 16 pixels x COB/8 vectors of 8 channels
+#pragma unroll
+            for (int k = 0; k < COB / 32; k++) {
+              const int idx = lane + 64 * k;
+              const int px = idx / (COB / 8), c8 = idx % (COB / 8);
+              const int x = x0 + px, co = co0 + 8 * c8;
+              if (y < a.H && x < a.W && co < a.Cout)
+                *reinterpret_cast<uint4*>(a.y + (((long)n * a.H + y) * a.W + x) * a.ldy + a.cy0 + co) =
+                    *reinterpret_cast<const uint4*>(&ep[px * COB + 8 * c8]);
+            }
+            wave_sync();
+          }
+        }
+      }
 #pragma unroll
       for (int j = 0; j < CT; j++) {
         const int co = co0 + 16 * j + r16;
@@ -231,7 +279,7 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int y = y0 + 4 * wave + i;
-          if (y >= a.H) continue;
+          if (vec_out || y >= a.H) continue;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const int x = x0 + 4 * g + r;
