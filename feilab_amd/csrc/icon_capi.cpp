@@ -694,7 +694,10 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   hipStream_t st = p->stream;
   p->data_sino = nullptr;   // the timed kernels overwrite the solve's data buffers
   RowFwdArgs rf{};
-  rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.cplx = 1; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
+  // the row pass with the ICON update fused, as in every iteration after the
+  // first; the image it updates lives in the (then stale) A^H W b buffer
+  rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.cplx = 0; rf.update = 2; rf.data_type = 1;
+  rf.m = reinterpret_cast<float*>(p->d_ahwb); rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
   rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
   ColArgs ca{};
   ca.K = p->g.K;
@@ -713,8 +716,8 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   sa.K = p->g.K;
   sa.rows = S * g.nz; sa.N0 = g.N0; sa.sino = reinterpret_cast<const float*>(p->d_b); sa.ck1 = p->d_ck1;
   sa.phi1 = p->d_phi1; sa.i10 = p->d_i10; sa.tw = p->d_tw0; sa.b = p->d_fw;
-  auto launch = [&]() -> hipError_t {
-    switch (kernel) {
+  auto launch = [&](int k) -> hipError_t {
+    switch (k) {
       case ICON_K_ROW_FWD: return launch_row_fwd(g.n1, rf, st);
       case ICON_K_COL_FWD: return launch_col(g.n0, p->bins.C, false, ca, st);
       case ICON_K_COMBINE: return launch_combine(cm, st);
@@ -724,33 +727,40 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
       default: return hipErrorInvalidValue;
     }
   };
-  // the kernel that precedes it in an iteration, so that each timed launch
-  // finds its input as the solve does (just written, partly in the caches)
-  auto before = [&]() -> hipError_t {
-    switch (kernel) {
-      case ICON_K_COL_FWD: return launch_row_fwd(g.n1, rf, st);
-      case ICON_K_COMBINE: return launch_col(g.n0, p->bins.C, false, ca, st);
-      case ICON_K_COL_ADJ: return launch_combine(cm, st);
-      case ICON_K_ROW_ADJ: return launch_col(g.n0, p->bins.C, true, ca, st);
-      case ICON_K_ROW_FWD: return launch_combine(cm, st);
-      default: return hipSuccess;
+  if (kernel < ICON_K_ROW_FWD || kernel > ICON_K_SINO) return fail(ICON_ERR_ARG, "icon_time_kernel: unknown kernel %d", kernel);
+  // The launch sequence of one iteration, as the solve enqueues it; the
+  // timed kernel runs inside it, so that each timed launch finds its input as
+  // in a solve (just written by its predecessor, partly in the caches) and the
+  // device busy (an idle gap before a launch lowers the clock it starts at).
+  // The sinogram transform runs once per solve and is timed back to back.
+  const int iter_seq[7] = {ICON_K_ROW_FWD, ICON_K_COL_FWD, ICON_K_COMBINE, ICON_K_COL_ADJ,
+                           ICON_K_ROW_ADJ, ICON_K_COL_FWD, ICON_K_COMBINE};
+  const int sino_seq[1] = {ICON_K_SINO};
+  const int* seq = kernel == ICON_K_SINO ? sino_seq : iter_seq;
+  const int nseq = kernel == ICON_K_SINO ? 1 : 7;
+  // Each timed launch carries events that take the kernel's own start and end
+  // (hipExtLaunchKernel), not those of extra marker packets around it; all
+  // launches are enqueued before one synchronisation.
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+  for (int i = 0; i < nseq && e == hipSuccess; i++) e = launch(seq[i]);  // untimed warm-up pass
+  for (int r = 0; r < reps && e == hipSuccess;) {
+    for (int i = 0; i < nseq && e == hipSuccess; i++) {
+      const bool timed = seq[i] == kernel && r < reps;
+      if (timed) set_launch_events(ev[2 * r], ev[2 * r + 1]);
+      e = launch(seq[i]);
+      if (timed) {
+        set_launch_events(nullptr, nullptr);
+        r++;
+      }
     }
-  };
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  hipError_t e = launch();  // warm
-  std::vector<float> t(reps, 0.f);
-  for (int r = 0; r < reps && e == hipSuccess; r++) {
-    e = before();
-    if (e == hipSuccess) e = hipEventRecord(e0, st);
-    if (e == hipSuccess) e = launch();
-    if (e == hipSuccess) e = hipEventRecord(e1, st);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&t[r], e0, e1);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  std::vector<float> t(reps, 0.f);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = hipEventElapsedTime(&t[r], ev[2 * r], ev[2 * r + 1]);
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
   if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "icon_time_kernel: %s", hipGetErrorString(e));
   // median launch duration: robust to a launch slowed by other work on the device
   std::sort(t.begin(), t.end());

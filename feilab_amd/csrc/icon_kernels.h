@@ -99,6 +99,12 @@ struct ReprojArgs {
   float* out;            // [S][N0]
 };
 
+// While set (non-null) on the calling thread, the next launches of the
+// iteration kernels (row_fwd, col, combine, row_adj, sino) go through
+// hipExtLaunchKernel with these events, which then hold the kernel's own start
+// and end on the device clock. Used only for timing, never under capture.
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
+
 hipError_t launch_row_fwd(int NR, const RowFwdArgs& a, hipStream_t st);
 hipError_t launch_row_adj(int NR, const RowAdjArgs& a, hipStream_t st);
 hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t st);

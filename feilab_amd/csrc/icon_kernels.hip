@@ -27,6 +27,7 @@
 //                  nufft_GPhi_1dCU :1454-1477
 //   k_reproj       cunfft_adjoint_1d :1518-1536 (1D gather in ascending node
 //                  order, inverse FFT, nufft_assignF_HAT_1d :1123-1135)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -706,12 +707,29 @@ __global__ __launch_bounds__(256) void k_copy_f4(float4* __restrict__ dst, const
 
 namespace {
 
+thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+
 template <int NR>
 constexpr int rows_per_block() {
   return NR >= 2048 ? 2 : 256 / (NR / 16) > 16 ? 16 : 256 / (NR / 16);
 }
 
 }  // namespace
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+  t_ev_start = start;
+  t_ev_stop = stop;
+}
+
+// hipLaunchKernelGGL, or, while timing events are set, hipExtLaunchKernelGGL
+// with them (the events then bracket the kernel itself, not the enqueue)
+#define LAUNCH_TIMED(KERNEL, GRID, BLOCK, SHM, ST, ...)                                          \
+  do {                                                                                           \
+    if (t_ev_start)                                                                              \
+      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, ST, t_ev_start, t_ev_stop, 0, __VA_ARGS__); \
+    else                                                                                         \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, ST, __VA_ARGS__);                             \
+  } while (0)
 
 #define ROW_CASES(X) X(128) X(256) X(512) X(1024) X(2048)
 
@@ -721,7 +739,7 @@ hipError_t launch_row_fwd(int NR, const RowFwdArgs& a, hipStream_t st) {
 #define CASE(N)                                                                                     \
   case N: {                                                                                         \
     constexpr int RPB = rows_per_block<N>();                                                        \
-    hipLaunchKernelGGL((k_row_fwd<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+    LAUNCH_TIMED((k_row_fwd<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
                        st, a);                                                                      \
     break;                                                                                          \
   }
@@ -739,10 +757,10 @@ hipError_t launch_row_adj(int NR, const RowAdjArgs& a, hipStream_t st) {
   case N: {                                                                                         \
     constexpr int RPB = rows_per_block<N>();                                                        \
     if (h.fwd_out)                                                                                  \
-      hipLaunchKernelGGL((k_row_adj<N, RPB, true>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+      LAUNCH_TIMED((k_row_adj<N, RPB, true>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
                          st, a);                                                                    \
     else                                                                                            \
-      hipLaunchKernelGGL((k_row_adj<N, RPB, false>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+      LAUNCH_TIMED((k_row_adj<N, RPB, false>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
                          st, a);                                                                    \
     break;                                                                                          \
   }
@@ -759,7 +777,7 @@ hipError_t launch_sino(int NR, const SinoArgs& a, hipStream_t st) {
 #define CASE(N)                                                                                  \
   case N: {                                                                                      \
     constexpr int RPB = rows_per_block<N>();                                                     \
-    hipLaunchKernelGGL((k_sino<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
+    LAUNCH_TIMED((k_sino<N, RPB>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
                        st, a);                                                                   \
     break;                                                                                       \
   }
@@ -859,10 +877,10 @@ hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t
     e = col_attr<N, CC, KK>();                                                                           \
     if (e != hipSuccess) return e;                                                                       \
     if (adjoint)                                                                                         \
-      hipLaunchKernelGGL((k_col_adj<N, CC, KK>), dim3(h.S * nblk), dim3(CC * (N / 16)),                  \
+      LAUNCH_TIMED((k_col_adj<N, CC, KK>), dim3(h.S * nblk), dim3(CC * (N / 16)),                  \
                          (col_lds<N, CC>()) * sizeof(float2), st, a);                                    \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_col_fwd<N, CC, KK>), dim3(h.S * nblk), dim3(CC * (N / 16)),                  \
+      LAUNCH_TIMED((k_col_fwd<N, CC, KK>), dim3(h.S * nblk), dim3(CC * (N / 16)),                  \
                          (col_lds<N, CC>()) * sizeof(float2), st, a);                                    \
     return hipGetLastError();                                                                            \
   }
@@ -888,7 +906,7 @@ hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int s
 }
 
 hipError_t launch_combine(const CombineArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_combine, dim3(a.S), dim3(1024), 0, st, a);
+  LAUNCH_TIMED(k_combine, dim3(a.S), dim3(1024), 0, st, a);
   return hipGetLastError();
 }
 

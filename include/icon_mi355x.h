@@ -201,10 +201,12 @@ typedef enum {
 } icon_kernel_id;
 
 /* Times `reps` launches of one kernel of the iteration over S slices, each
- * right after the kernel that precedes it in an iteration (untimed), with
- * HIP events around the timed launch on the plan's stream, on the buffers of
- * the plan's last solve; *avg_ms = median launch duration.  Not part of the
- * reference; measurement only. */
+ * right after the kernel that precedes it in an iteration (untimed), all
+ * enqueued back to back on the plan's stream after 3 untimed pairs, on the
+ * buffers of the plan's last solve.  Each timed launch carries HIP events
+ * that take the kernel's own start and end (hipExtLaunchKernel);
+ * *avg_ms = median launch duration.  Not part of the reference; measurement
+ * only. */
 int icon_time_kernel(icon_plan* plan, int kernel, int S, int reps, double* avg_ms);
 
 /* Device-to-device copy of `bytes` repeated `reps` times with a plain vector
