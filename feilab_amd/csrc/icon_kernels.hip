@@ -232,6 +232,9 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int h0 = a.N0 >> 1;
+  // a tile no node reaches (beyond the largest |ky| of the tilt range) has
+  // nothing to interpolate: its FFT is not needed
+  if (a.ent_off[blk] == a.ent_off[blk + 1]) return;
   // x FFT of column c = tid % C: the first pass takes its 16 inputs (grid
   // rows t + r NC/16; zero padding outside the N0 non-zero rows) straight
   // from global memory, all loads in flight together; the transform leaves
@@ -326,6 +329,12 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   const int s = blockIdx.x % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
+  // a tile no node reaches: its spread grid and so its transform are zero
+  if (a.ent_off[blk] == a.ent_off[blk + 1]) {
+    for (int i = tid; i < a.N0 * C; i += NT)
+      a.out[((size_t)s * a.N0 + i / C) * a.n1 + y0 + i % C] = make_float2(0.f, 0.f);
+    return;
+  }
   // Spread as a gather.  Thread tid owns the RL adjacent grid rows
   // [x0, x0+RL) of the tile (C cells each, in registers).  The entries that
   // cover them are those with first x-tap z in [x0-13, x0+RL-1] (mod n0): one
