@@ -283,9 +283,7 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
     // separable window: each grid row is first reduced over its C y-taps,
     // then weighted by its x-tap
     float ax = 0.f, ay = 0.f;
-#pragma unroll 2
-    for (int q = 0; q < K; q++) {
-      const float2* row = tile + ((en.z + q) & (NC - 1)) * LS;
+    auto add_row = [&](const float2* row, int q) {
       float2 gv[C];
       if constexpr (LS % 2 == 0) {
 #pragma unroll
@@ -306,6 +304,17 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
       }
       ax = __fmaf_rn(tx, px[q], ax);
       ay = __fmaf_rn(ty, px[q], ay);
+    };
+    // The entries are sorted by first x-tap, so only the last wavefronts of
+    // a tile can hold taps that wrap past row NC-1; the others take the
+    // fully unrolled loop with one base address and constant row offsets.
+    if (__builtin_amdgcn_ballot_w64(en.z > NC - K) == 0) {
+      const float2* base = tile + en.z * LS;
+#pragma unroll
+      for (int q = 0; q < K; q++) add_row(base + q * LS, q);
+    } else {
+#pragma unroll 2
+      for (int q = 0; q < K; q++) add_row(tile + ((en.z + q) & (NC - 1)) * LS, q);
     }
     a.part[((size_t)s * a.M + en.x) * kMaxSlotK + ((en.y >> 16) & 0xff)] = make_float2(ax, ay);
   }
