@@ -10,7 +10,7 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   Its per-slice data residual R(m) = sqrt(sum w|Am-b|^2 / sum w|b|^2) (FP64)
   is the target, its reprojection FRC at the min-|theta| tilt the reference.
 * Learned leg: a U-Net trained on the device (its own BF16 MFMA kernels) on
-  (10-iteration, 200-iteration) pairs of a separate phantom; one step = 10
+  (4-iteration, 200-iteration) pairs of a separate phantom; one step = 4
   plain iterations, the U-Net warm start, and k descent iterations, with k the
   smallest continuation that brings every slice to R <= its plain target with
   the reprojection FRC (rings 1..N/4) within 0.005 of plain ICON's.  Slices
@@ -172,7 +172,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--iters", default="20,160,20", help="pre,icon,post")
-    ap.add_argument("--start", type=int, default=10, help="plain iterations before the U-Net")
+    ap.add_argument("--start", type=int, default=4, help="plain iterations before the U-Net")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-slices", type=int, default=256)
     ap.add_argument("--unet-batch", type=int, default=32)
