@@ -142,7 +142,8 @@ int refresh_bf16(icon_unet* n, hipStream_t st) {
 template <typename TX, typename TY>
 int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int ldy, int cy0, int B, int H, int W,
          bool relu, const uint16_t* mask, int ldm, int cm0, bool flipped, bool accumulate, hipStream_t st,
-         const T16* up = nullptr, const T16* pool = nullptr) {
+         const T16* up = nullptr, const T16* pool = nullptr, const float* head_w = nullptr,
+         const float* head_b = nullptr, float* head_out = nullptr) {
   ConvArgs<TX, TY> a{};
   a.x = x; a.ldx = ldx; a.cx0 = cx0;
   a.w = flipped ? c.wt16 : c.w16;
@@ -156,6 +157,7 @@ int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int 
   a.relu = relu; a.accumulate = accumulate;
   if (up) { a.up = up->p; a.ldu = up->C; a.cu0 = 0; a.Cup = up->C; }
   if (pool) { a.pool = pool->p; a.ldp = pool->C; a.cp0 = 0; }
+  a.head_w = head_w; a.head_b = head_b; a.head_out = head_out;
   UCHK(launch_conv(a, st));
   return 0;
 }
@@ -217,9 +219,16 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   if (!fuse_up) R(up(n->d2b, n->cat1, 0, B, H2, W2, st));
   R(conv(n, cv[D1A], n->cat1.p, 96, 0, n->d1a.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st,
          fuse_up ? &n->d2b : nullptr));
-  R(conv(n, cv[D1B], n->d1a.p, 32, 0, n->d1b.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
-  if (launch_head_fwd(n->d1b.p, 32, n->params + n->head_w, n->params + n->head_b, 32, P, d_out, st) != hipSuccess)
-    return ufail(2, "head");
+  if (n->train) {
+    R(conv(n, cv[D1B], n->d1a.p, 32, 0, n->d1b.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
+    if (launch_head_fwd(n->d1b.p, 32, n->params + n->head_w, n->params + n->head_b, 32, P, d_out, st) != hipSuccess)
+      return ufail(2, "head");
+  } else {
+    // inference: the head in D1B's epilogue (the same sums on the same BF16
+    // values), D1B's output is not stored
+    R(conv(n, cv[D1B], n->d1a.p, 32, 0, n->d1b.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st, nullptr,
+           nullptr, n->params + n->head_w, n->params + n->head_b, d_out));
+  }
 #undef R
   return 0;
 }

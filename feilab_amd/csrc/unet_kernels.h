@@ -30,6 +30,13 @@ struct ConvArgs {
   // (H/2 x W/2 pixels)
   uint16_t* pool;
   int ldp, cp0;
+  // optional (BF16 output, Cout = 32, inference): the 1x1 head fused into the
+  // epilogue, head_out[p] = head_b + sum_c head_w[c] y[p][c] over the BF16
+  // values of y in channel order (what k_head_fwd computes); y itself is not
+  // stored
+  const float* head_w;
+  const float* head_b;
+  float* head_out;
 };
 
 // dw[tap][co][ci] += sum over pixels of dy[p][co] * x[p + offset(tap)][ci]

@@ -46,6 +46,18 @@ union V8 {
   bf8 v;
 };
 
+// Sum of the 1x1 head over one group of 8 channels, in channel order,
+// starting from s0.  The head adds the groups' sums in group order:
+// out = ((b + S_0) + S_1) + ... with S_g the group sum started from 0 (g > 0),
+// the order that the fused head of the convolution epilogue also follows.
+__device__ __forceinline__ float head_group(const V8& v, const float* w, float s0) {
+  float s = s0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) s = fmaf(bf2f(v.h[k]), w[k], s);
+  return s;
+}
+
+
 // 8 consecutive channels of one pixel as a BF16 vector (zeros if !ok).
 template <typename T>
 __device__ __forceinline__ bf8 load8(const T* p, bool ok);
@@ -257,15 +269,28 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
                 ep[(4 * g + r) * COB + 16 * j + r16] = f2bf(v);
               }
             wave_sync();
-            // 16 pixels x COB/8 vectors of 8 channels
+            if (a.head_out) {
+              // the head of the row tile's 16 pixels: lane = pixel + 16 x
+              // group of 8 channels (COB = Cout = 32); the group sums are
+              // added in group order, as k_head_fwd adds them
+              const int hp = lane & (kTile - 1), hg = lane >> 4;
+              V8 v;
+              v.u = *reinterpret_cast<const uint4*>(&ep[hp * COB + 8 * hg]);
+              const float part = head_group(v, a.head_w + 8 * hg, hg == 0 ? a.head_b[0] : 0.f);
+              const float p1 = __shfl(part, hp + 16), p2 = __shfl(part, hp + 32), p3 = __shfl(part, hp + 48);
+              const int x = x0 + hp;
+              if (hg == 0 && y < a.H && x < a.W) a.head_out[((long)n * a.H + y) * a.W + x] = ((part + p1) + p2) + p3;
+            } else {
+              // 16 pixels x COB/8 vectors of 8 channels
 #pragma unroll
-            for (int k = 0; k < COB / 32; k++) {
-              const int idx = lane + 64 * k;
-              const int px = idx / (COB / 8), c8 = idx % (COB / 8);
-              const int x = x0 + px, co = co0 + 8 * c8;
-              if (y < a.H && x < a.W && co < a.Cout)
-                *reinterpret_cast<uint4*>(a.y + (((long)n * a.H + y) * a.W + x) * a.ldy + a.cy0 + co) =
-                    *reinterpret_cast<const uint4*>(&ep[px * COB + 8 * c8]);
+              for (int k = 0; k < COB / 32; k++) {
+                const int idx = lane + 64 * k;
+                const int px = idx / (COB / 8), c8 = idx % (COB / 8);
+                const int x = x0 + px, co = co0 + 8 * c8;
+                if (y < a.H && x < a.W && co < a.Cout)
+                  *reinterpret_cast<uint4*>(a.y + (((long)n * a.H + y) * a.W + x) * a.ldy + a.cy0 + co) =
+                      *reinterpret_cast<const uint4*>(&ep[px * COB + 8 * c8]);
+              }
             }
             wave_sync();
           }
@@ -492,15 +517,15 @@ __global__ void k_relu_mask(float* g, int ldg, int cg0, const uint16_t* act, int
   }
 }
 
-// 1x1 head: out[p] = sum_c x[p][c] w[c] + b  (fp32 out); C a multiple of 8
+// 1x1 head: out[p] = b + sum_c x[p][c] w[c]  (fp32 out); C a multiple of 8
 __global__ void k_head_fwd(const uint16_t* x, int ldx, const float* w, const float* b, int C, long P, float* out) {
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
-    float s = b[0];
+    float s = 0.f;
     for (int c = 0; c < C; c += 8) {
       V8 v;
       v.u = *reinterpret_cast<const uint4*>(x + p * ldx + c);
-#pragma unroll
-      for (int k = 0; k < 8; k++) s = fmaf(bf2f(v.h[k]), w[c + k], s);
+      const float g = head_group(v, w + c, c == 0 ? b[0] : 0.f);
+      s = c == 0 ? g : s + g;
     }
     out[p] = s;
   }
@@ -674,6 +699,10 @@ static int blocks_for(long n, int per = 256) { return (int)std::min<long>((n + p
 template <typename TX, typename TY>
 hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
   if (a.taps != 9 && a.taps != 1) return hipErrorInvalidValue;
+  // the fused head needs all 32 output channels in one block's BF16 epilogue
+  if (a.head_out && (sizeof(TY) != 2 || a.Cout != 32 || a.mask || a.accumulate || a.pool || a.ldy % 8 ||
+                     a.cy0 % 8 || !a.head_w || !a.head_b))
+    return hipErrorInvalidValue;
   const long tiles = (long)a.N * ((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
   // persistent grid: the blocks that fit the device at once (LDS-bound:
   // 3 per CU at COB 32, 2 at COB 64), over all output-channel blocks
