@@ -257,17 +257,26 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
             const int co = co0 + 16 * j + r16;
             bias[j] = a.bias && co < a.Cout ? a.bias[co] : 0.f;
           }
+          // bias, ReLU and the rounding to BF16 once, in place: the stores
+          // and the pool below both read the rounded values
 #pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const int y = y0 + 4 * wave + i;
+          for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < CT; j++)
 #pragma unroll
               for (int r = 0; r < 4; r++) {
                 float v = acc[i][j][r] + bias[j];
                 if (a.relu) v = fmaxf(v, 0.f);
-                ep[(4 * g + r) * COB + 16 * j + r16] = f2bf(v);
+                acc[i][j][r] = bf2f(f2bf(v));
               }
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int y = y0 + 4 * wave + i;
+#pragma unroll
+            for (int j = 0; j < CT; j++)
+#pragma unroll
+              for (int r = 0; r < 4; r++)
+                ep[(4 * g + r) * COB + 16 * j + r16] = (uint16_t)(__float_as_uint(acc[i][j][r]) >> 16);
             wave_sync();
             if (a.head_out) {
               // the head of the row tile's 16 pixels: lane = pixel + 16 x
@@ -334,9 +343,13 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
                 float q4[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                  float v = acc[i + (k >> 1)][j][r + (k & 1)] + bias;
-                  if (a.relu) v = fmaxf(v, 0.f);
-                  q4[k] = bf2f(f2bf(v));
+                  if (vec_out) {
+                    q4[k] = acc[i + (k >> 1)][j][r + (k & 1)];   // already biased, ReLU'd and rounded
+                  } else {
+                    float v = acc[i + (k >> 1)][j][r + (k & 1)] + bias;
+                    if (a.relu) v = fmaxf(v, 0.f);
+                    q4[k] = bf2f(f2bf(v));
+                  }
                 }
                 const long pp = ((long)n * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
                 a.pool[pp * a.ldp + a.cp0 + co] = f2bf(0.25f * (q4[0] + q4[1] + q4[2] + q4[3]));
