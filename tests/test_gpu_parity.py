@@ -144,6 +144,22 @@ def test_icon_solve_config1_vs_oracle(icon, oracle_mod):
     assert abs(res[-1] - reso[-1]) / reso[-1] < 1e-4
 
 
+def test_bench_geometry_solve_vs_oracle(icon, oracle_mod):
+    """The benchmark's geometry (512^2, 41 tilts +-60 step 3, a tilt range
+    that leaves 14 of the 128 interpolation tiles without nodes) on two
+    slices of the benchmark phantom, 20 iterations, against the FP32 mirror."""
+    cfg = phantom.config2(nslices=2, nblobs=40)
+    _print("solve bench geometry 512x512 x 41 tilts, 2 slices")
+    plan = icon.Plan(512, cfg["tilt"], max_slices=2)
+    orc = oracle_mod.Plan(512, 512, cfg["tilt"], 32)
+    m, res, _ = plan.solve(cfg["sino"], 5, 10, 5, 2, 0.0, trace=True)
+    for k in range(2):
+        mo, reso, _ = orc.solve(cfg["sino"][k], 5, 10, 5, 2, 0.0, trace=True)
+        _print("bench geometry slice", k, "rel", rel(m[k], mo), "resid", res[k][-1], reso[-1])
+        assert rel(m[k], mo) < 1e-4
+        assert abs(res[k][-1] - reso[-1]) / reso[-1] < 1e-4
+
+
 def test_batch_matches_single(icon):
     """Slices of a batch are independent: solving them together gives each
     slice's single-slice result."""
