@@ -33,6 +33,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "512² slices/s to fixed data residual, learned-start vs plain ICON, 1 GPU"
+
+# U-Net convolutions (Cin, Cout, side at a 512^2 input) for the forward FLOP count
+UNET_CONVS = [(1, 32, 512), (32, 32, 512), (32, 64, 256), (64, 64, 256), (64, 128, 128), (128, 128, 128),
+              (128, 256, 64), (256, 256, 64), (384, 128, 128), (128, 128, 128), (192, 64, 256), (64, 64, 256),
+              (96, 32, 512), (32, 32, 512)]
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def unet_gflop_per_slice():
+    """Forward FLOPs of the U-Net on one 512^2 slice: 3x3 convolutions and the
+    1x1 head, 2 FLOP per multiply-add (pool, upsample and ReLU not counted)."""
+    return (sum(2.0 * h * h * ci * co * 9 for ci, co, h in UNET_CONVS) + 2.0 * 512 * 512 * 32) / 1e9
 HBM_PEAK_GBS = 8000.0
 
 
@@ -303,6 +315,7 @@ def main():
 
     # ---- learned leg
     learned = None
+    unet_roofline = None
     value, ms_step = plain_value, dt_plain / args.steps * 1e3
     if not args.no_learned:
         log("training the warm start: %d slices of a separate phantom, %d steps" % (args.train_slices, args.train_steps))
@@ -357,6 +370,16 @@ def main():
                            train_loss_last=round(float(np.mean(losses[-50:])), 4), train_seconds=round(train_s, 1),
                            unet_batch=args.unet_batch, no_network=nonet)
             log("learned:", json.dumps(learned))
+            # the U-Net forward alone (the largest part of a learned step)
+            dt_u = timed(lambda: solver.net.warm_start_dev(S, solver.d_start, solver.d_init), "unet forward")
+            gf = unet_gflop_per_slice()
+            tf = gf * S * args.steps / dt_u / 1e3
+            unet_roofline = dict(bound="mfma", achieved=round(tf, 1), peak=BF16_DENSE_PEAK_TFLOPS, unit="TFLOP/s",
+                                 frac=round(tf / BF16_DENSE_PEAK_TFLOPS, 4),
+                                 kernel="U-Net forward, all layers (BF16 MFMA convolutions), batches of %d" % args.unet_batch,
+                                 ms_per_256_slices=round(dt_u / args.steps * 1e3 * 256 / S, 2),
+                                 gflop_per_slice=round(gf, 2))
+            log("unet forward:", json.dumps(unet_roofline))
         solver.close()
 
     cpu = None
@@ -396,6 +419,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "iteration_roofline": iteration_roofline(plan, S, per_iter_ms),
+        "unet_roofline": unet_roofline,
         "kb6_variant": kb6,
     }
     print(json.dumps(out), flush=True)
