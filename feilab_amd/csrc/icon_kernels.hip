@@ -213,6 +213,16 @@ __host__ __device__ constexpr int col_stride() {
 template <int C>
 using ColFftLay = Lay<C + 1, 4, 8>;
 
+// The interpolation kernel's x FFT uses row stride C instead: with 8 float2
+// of padding after every 16 rows, the 64/C rows a wavefront's loads and
+// stores touch at once start 16 banks apart (2 cycles per 64-lane 8-byte
+// access, the minimum) where stride C + 1 makes every load and the second
+// exchange's stores take 3.  The spread kernel keeps C + 1: its threads
+// write whole rows, which stride C would put on the same banks.
+template <int C>
+using ColFftLayFwd = Lay<C, 4, 8>;
+static_assert(ColFftLayFwd<8>::at(1023) <= ColFftLay<8>::at(1023), "the forward layout fits the spread layout's LDS");
+
 // float2 of LDS per column block: the tile, or the FFT layout if larger
 template <int NC, int C>
 __host__ __device__ constexpr int col_lds() {
@@ -253,7 +263,7 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
       const int X = (x + h0) & (NC - 1);
       if (X < a.N0) v[r] = a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c];
     }
-    fft_regs_half<NC, ColFftLay<C>, -1>(v, tile + c, t, T);
+    fft_regs_half<NC, ColFftLayFwd<C>, -1>(v, tile + c, t, T);
     // the last exchange ended with a barrier: the column can be overwritten
 #pragma unroll
     for (int i = 0; i < 16; i++) tile[out_index<NC>(t, i) * LS + c] = v[i];
