@@ -206,28 +206,27 @@ __host__ __device__ constexpr int col_stride() {
   return NC * (C + 2) * 8 <= 81920 ? C + 2 : C + 1;
 }
 
-// Layout of the tile during the x FFT's exchanges: row stride C + 1 and 8
-// float2 of padding after every 16 rows.  A wavefront holds 64/C threads
-// per column whose first-pass outputs start 16 rows apart; the padding puts
-// them on different banks (the unpadded tile would conflict 64/C ways).
-template <int C>
-using ColFftLay = Lay<C + 1, 4, 8>;
-
-// The interpolation kernel's x FFT uses row stride C instead: with 8 float2
-// of padding after every 16 rows, the 64/C rows a wavefront's loads and
-// stores touch at once start 16 banks apart (2 cycles per 64-lane 8-byte
-// access, the minimum) where stride C + 1 makes every load and the second
-// exchange's stores take 3.  The spread kernel keeps C + 1: its threads
-// write whole rows, which stride C would put on the same banks.
+// Layouts of the tile during the x FFT's exchanges.  A wavefront holds 64/C
+// threads per column; at C = 8 each of its 8-byte loads and stores touches 8
+// rows of the C columns at once, and the layout puts those rows on disjoint
+// banks (2 cycles per 64-lane access, the minimum):
+// * interpolation: row stride C, 8 float2 of padding after every 16 rows;
+// * spread: row stride C, 1 float2 of padding after every 4 rows, which also
+//   keeps the spread's own stores conflict-free (each thread writes its two
+//   whole rows, 2 apart across lanes, where the padding every 16 rows would
+//   put 4 lanes on one bank).
+// The stride C + 1 used before took 3 cycles for most FFT accesses.
 template <int C>
 using ColFftLayFwd = Lay<C, 4, 8>;
-static_assert(ColFftLayFwd<8>::at(1023) <= ColFftLay<8>::at(1023), "the forward layout fits the spread layout's LDS");
+template <int C>
+using ColFftLay = Lay<C, 2, 1>;
 
-// float2 of LDS per column block: the tile, or the FFT layout if larger
+// float2 of LDS per column block: the tile, or an FFT layout if larger
 template <int NC, int C>
 __host__ __device__ constexpr int col_lds() {
-  return NC * col_stride<NC, C>() > ColFftLay<C>::at(NC - 1) + C ? NC * col_stride<NC, C>()
-                                                                 : ColFftLay<C>::at(NC - 1) + C;
+  constexpr int tile = NC * col_stride<NC, C>();
+  constexpr int fwd = ColFftLayFwd<C>::at(NC - 1) + C, adj = ColFftLay<C>::at(NC - 1) + C;
+  return tile > fwd ? (tile > adj ? tile : adj) : (fwd > adj ? fwd : adj);
 }
 
 template <int NC, int C, int K>
