@@ -119,8 +119,12 @@ void free_bins(icon_plan* p) {
   if (p->d_ent_py) (void)hipFree(p->d_ent_py);
   if (p->d_ent_of_node) (void)hipFree(p->d_ent_of_node);
   if (p->d_fw) (void)hipFree(p->d_fw);
+  if (p->d_fent) (void)hipFree(p->d_fent);
+  if (p->d_fent_px) (void)hipFree(p->d_fent_px);
+  if (p->d_fent_py) (void)hipFree(p->d_fent_py);
   p->d_ent_off = nullptr; p->d_ent = nullptr; p->d_nslot = nullptr; p->d_xoff = nullptr;
   p->d_ent_px = nullptr; p->d_ent_py = nullptr; p->d_ent_of_node = nullptr; p->d_fw = nullptr;
+  p->d_fent = nullptr; p->d_fent_px = nullptr; p->d_fent_py = nullptr;
 }
 
 int upload_bins(icon_plan* p) {
@@ -132,6 +136,9 @@ int upload_bins(icon_plan* p) {
   if ((rc = upload(p, &p->d_ent_px, p->bins.px))) return rc;
   if ((rc = upload(p, &p->d_ent_py, p->bins.py))) return rc;
   if ((rc = upload(p, &p->d_ent_of_node, p->bins.of_node))) return rc;
+  if ((rc = upload(p, &p->d_fent, p->bins.fent))) return rc;
+  if ((rc = upload(p, &p->d_fent_px, p->bins.fpx))) return rc;
+  if ((rc = upload(p, &p->d_fent_py, p->bins.fpy))) return rc;
   // weighted node value per bin entry, per slice
   if ((rc = dalloc(p, &p->d_fw, (size_t)p->S_max * p->bins.ent.size()))) return rc;
   return ICON_OK;
@@ -183,6 +190,7 @@ int op_forward(icon_plan* p, int S, const void* img, int cplx, float2* out, hipS
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
   ca.in = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.part = p->d_part;
   if ((rc = check_launch(launch_col(g.n0, p->bins.C, false, ca, st), "col_fwd"))) return rc;
   CombineArgs cb{};
@@ -199,6 +207,7 @@ int op_adjoint_ent(icon_plan* p, int S, float2* img, hipStream_t st) {
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
   ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.fw = f;
   int rc = check_launch(launch_col(g.n0, p->bins.C, true, ca, st), "col_adj");
   if (rc) return rc;
@@ -256,7 +265,8 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
-  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
+  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
   CombineArgs cm{};
   cm.S = S; cm.M = g.M; cm.N0 = g.N0; cm.nz = g.nz; cm.mode = 0; cm.part = p->d_part; cm.nslot = p->d_nslot;
@@ -702,7 +712,8 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
-  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
+  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
   CombineArgs cm{};
   cm.S = S; cm.M = g.M; cm.N0 = g.N0; cm.nz = g.nz; cm.mode = 0; cm.part = p->d_part; cm.nslot = p->d_nslot;

@@ -334,6 +334,33 @@ void build_bins(const Geometry& g, int C, Bins& b) {
       b.of_node[(size_t)j * kMaxSlot + slot] = (int)e;
     }
   }
+  // Interpolation order.  The interpolation reads rows of 80 bytes from the
+  // LDS tile, rows x0 .. x0+K-1 of entry e in lane e; two lanes of a 16-lane
+  // group whose first taps differ by a multiple of 16 rows hit the same
+  // banks.  Each group of 16 takes the next entry (by x0) of every residue
+  // x0 mod 16 in turn, so its first taps differ mod 16 while entries remain
+  // in 16 residues, and stay close to each other.
+  b.fent.resize(E);
+  b.fpx.assign(E * K, 0.f);
+  b.fpy.assign(E * (size_t)C, 0.f);
+  for (int blk = 0; blk < b.nblk; blk++) {
+    const size_t e0 = b.off[blk], ne = b.off[blk + 1] - e0;
+    std::vector<size_t> bucket[16];
+    for (size_t e = e0; e < e0 + ne; e++) bucket[b.ent[e].z & 15].push_back(e);
+    size_t pos[16] = {}, f = e0;
+    while (f < e0 + ne) {
+      for (int r = 0; r < 16; r++) {
+        if (pos[r] == bucket[r].size()) continue;
+        const size_t e = bucket[r][pos[r]++];
+        b.fent[f] = b.ent[e];
+        for (int t = 0; t < K; t++)
+          b.fpx[e0 * K + ((t / 2) * ne + (f - e0)) * 2 + (t & 1)] = b.px[e0 * K + ((t / 2) * ne + (e - e0)) * 2 + (t & 1)];
+        for (int c = 0; c < C; c++)
+          b.fpy[e0 * C + ((c / 4) * ne + (f - e0)) * 4 + (c & 3)] = b.py[e0 * C + ((c / 4) * ne + (e - e0)) * 4 + (c & 3)];
+        f++;
+      }
+    }
+  }
 }
 
 }  // namespace icon

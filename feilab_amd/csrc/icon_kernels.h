@@ -51,6 +51,10 @@ struct ColArgs {
   const int* ix0;
   const float* ent_px;     // [E][K], interleaved per tile
   const float* ent_py;     // [E][C], interleaved per tile
+  // fwd: the entries and their records in the interpolation's order
+  const int4* fent;
+  const float* fent_px;
+  const float* fent_py;
   float2* part;            // fwd: [S][M][3]
   const float2* fw;        // adj: [S][E] weighted value per bin entry
 };

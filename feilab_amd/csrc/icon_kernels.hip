@@ -273,9 +273,9 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   // vector loads and the thread sums its 14 x C cells from the LDS tile
   const int e0 = a.ent_off[blk], e1 = a.ent_off[blk + 1], ne = e1 - e0;
   for (int e = e0 + tid; e < e1; e += NT) {
-    const int4 en = a.ent[e];
+    const int4 en = a.fent[e];
     float px[K];
-    const float2* pxv = tile_px<K>(a, e0) + (e - e0);
+    const float2* pxv = reinterpret_cast<const float2*>(a.fent_px + (size_t)e0 * K) + (e - e0);
 #pragma unroll
     for (int q = 0; q < K / 2; q++) {
       const float2 p = pxv[q * ne];
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
       px[2 * q + 1] = p.y;
     }
     float py[C];
-    const float4* pyv = tile_py<C>(a, e0) + (e - e0);
+    const float4* pyv = reinterpret_cast<const float4*>(a.fent_py + (size_t)e0 * C) + (e - e0);
 #pragma unroll
     for (int c4 = 0; c4 < C / 4; c4++) {
       const float4 v = pyv[c4 * ne];
@@ -314,9 +314,10 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
       ax = __fmaf_rn(tx, px[q], ax);
       ay = __fmaf_rn(ty, px[q], ay);
     };
-    // The entries are sorted by first x-tap, so only the last wavefronts of
-    // a tile can hold taps that wrap past row NC-1; the others take the
-    // fully unrolled loop with one base address and constant row offsets.
+    // Entries whose taps wrap past row NC-1 have the largest first taps and
+    // come last in the tile's order: only the last wavefronts can hold them;
+    // the others take the fully unrolled loop with one base address and
+    // constant row offsets.
     if (__builtin_amdgcn_ballot_w64(en.z > NC - K) == 0) {
       const float2* base = tile + en.z * LS;
 #pragma unroll

@@ -49,6 +49,11 @@ struct Bins {
   std::vector<float> px;          // E*14: x-taps of the entry's node (interleaved per tile)
   std::vector<float> py;          // E*C: y-tap of each tile column, 0 outside the run (interleaved per tile)
   std::vector<int> of_node;       // M*3: entries of node j by slot (-1: none)
+  // The same entries and records in the interpolation's order: within a tile,
+  // groups of 16 entries whose first x-taps differ mod 16 where possible
+  // (records interleaved by field per tile as above).
+  std::vector<int4> fent;
+  std::vector<float> fpx, fpy;
 };
 
 // Fills g; returns false (with a message) on an invalid geometry.
@@ -100,6 +105,9 @@ struct icon_plan {
   int* d_xoff = nullptr;
   float* d_ent_px = nullptr;
   float* d_ent_py = nullptr;
+  int4* d_fent = nullptr;
+  float* d_fent_px = nullptr;
+  float* d_fent_py = nullptr;
   int* d_ent_of_node = nullptr;
   int8_t* d_nslot = nullptr;
   float* d_phi1 = nullptr;
