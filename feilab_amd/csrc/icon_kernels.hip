@@ -742,6 +742,15 @@ constexpr int rows_per_block() {
   return NR >= 2048 ? 2 : 256 / (NR / 16) > 16 ? 16 : 256 / (NR / 16);
 }
 
+// The adjoint row kernel (two transforms per row, a block barrier between
+// every exchange) runs in blocks of 128 threads: fewer wavefronts wait at
+// each barrier (0.833 vs 0.874 ms at 256 x 512 rows of 1024 against 256
+// threads; 64 threads: 0.840 ms).
+template <int NR>
+constexpr int row_adj_rows_per_block() {
+  return 128 / (NR / 16) > 1 ? 128 / (NR / 16) : 1;
+}
+
 }  // namespace
 
 void set_launch_events(hipEvent_t start, hipEvent_t stop) {
@@ -783,7 +792,7 @@ hipError_t launch_row_adj(int NR, const RowAdjArgs& a, hipStream_t st) {
   switch (NR) {
 #define CASE(N)                                                                                     \
   case N: {                                                                                         \
-    constexpr int RPB = rows_per_block<N>();                                                        \
+    constexpr int RPB = row_adj_rows_per_block<N>();                                                \
     if (h.fwd_out)                                                                                  \
       LAUNCH_TIMED((k_row_adj<N, RPB, true>), dim3((h.rows + RPB - 1) / RPB), dim3(RPB * (N / 16)), 0, \
                          st, a);                                                                    \
