@@ -187,7 +187,7 @@ def main():
     ap.add_argument("--start", type=int, default=4, help="plain iterations before the U-Net")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-slices", type=int, default=256)
-    ap.add_argument("--unet-batch", type=int, default=32)
+    ap.add_argument("--unet-batch", type=int, default=128, help="slices per U-Net forward launch sequence")
     ap.add_argument("--max-cont", type=int, default=40, help="continuation iterations tried")
     ap.add_argument("--frc-tol", type=float, default=0.005)
     ap.add_argument("--no-learned", action="store_true")
