@@ -113,7 +113,7 @@ def train_warm_start(plan, sino, start, schedule, steps=1500, patch=128, batch=1
 class LearnedSolver:
     """Learned-start reconstruction of up to max_slices slices on a plan."""
 
-    def __init__(self, plan, params, max_slices, start=4, data_type=2, threshold=0.0, device=0, batch=16):
+    def __init__(self, plan, params, max_slices, start=4, data_type=2, threshold=0.0, device=0, batch=128):
         self.plan, self.start, self.data_type, self.threshold = plan, start, data_type, threshold
         self.net = UNet(plan.N0, plan.N1, max_batch=min(batch, max_slices), device=device)
         self.net.set_params(params)
