@@ -111,12 +111,24 @@ def iteration_roofline(plan, S, per_iter_ms):
 
 def probe(what, S):
     """Child-process mode for the counter passes: run the named ICON kernel on
-    the workload's buffers (or a 1 GiB plain copy) a few times and exit."""
+    the workload's buffers (or a 1 GiB plain copy, or the U-Net forward) a few
+    times and exit."""
     import feilab_amd
     from feilab_amd import phantom
     from feilab_amd.icon import DeviceArray, SolveOptions, copy_probe
     if what == "copy":
         copy_probe(COPY_BYTES, 2)
+        return
+    if what == "unet":
+        # the U-Net forward of the learned leg (random weights), twice
+        from feilab_amd.unet import UNet
+        net = UNet(512, 512, max_batch=min(128, S), seed=1)
+        d_in = DeviceArray.from_host(np.random.default_rng(0).standard_normal((S, 512, 512)).astype(np.float32))
+        d_out = DeviceArray((S, 512, 512), np.float32)
+        for _ in range(2):
+            net.warm_start_dev(S, d_in, d_out)
+        feilab_amd.lib().icon_device_synchronize()
+        net.close()
         return
     cfg = phantom.config2(nslices=S)
     plan = feilab_amd.Plan(cfg["N"], cfg["tilt"], max_slices=S, device=0)
