@@ -47,6 +47,63 @@ template <int NR>
 constexpr int kRowLds = RowLay::size(NR);
 
 
+// Inputs 0..3 and 12..15 of thread t's first pass over image row X of slice
+// s (the others are padding): CPLX, the complex image rhat; otherwise the
+// real image m, with the ICON update m <- (float)(m - alpha Re(rhat)) and its
+// threshold applied and stored first (UPD).  All loads are issued before any
+// value is used (clamped to column 0 where the input is padding, the value
+// then dropped): a load under a per-lane condition would be compiled into a
+// branch whose merge waits for it before the next load is issued.
+template <int NR, bool CPLX, bool UPD>
+__device__ __forceinline__ void row_fwd_inputs(float2 (&v)[16], const RowFwdArgs& a, int t, int s, int X,
+                                               bool valid, int h1) {
+  float cj[8], mj[8];
+  float2 rj[8];
+  bool ok[8];
+  int Yj[8];
+  const size_t row0 = ((size_t)s * a.N0 + X) * a.N1;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int Y = (in_index<NR>(t, j < 4 ? j : j + 8) + h1) & (NR - 1);
+    ok[j] = valid && Y < a.N1;
+    Yj[j] = ok[j] ? Y : 0;
+    cj[j] = a.ck[(size_t)X * a.N1 + Yj[j]];
+    if constexpr (!CPLX) mj[j] = a.m[row0 + Yj[j]];
+    if constexpr (CPLX || UPD) rj[j] = a.rhat[row0 + Yj[j]];
+  }
+  const double al = UPD ? a.alpha[s] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    asm volatile("" : "+v"(cj[j]));
+    if constexpr (!CPLX) asm volatile("" : "+v"(mj[j]));
+    if constexpr (CPLX || UPD) asm volatile("" : "+v"(rj[j].x), "+v"(rj[j].y));
+  }
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    if (!ok[j]) continue;
+    const int r = j < 4 ? j : j + 8;
+    if constexpr (!CPLX) {
+      float mv = mj[j];
+      if constexpr (UPD) {
+        // m - al*Re(rhat) in double, stored as float, then the threshold
+        float nv = (float)__fma_rn(-al, (double)rj[j].x, (double)mv);
+        if (a.update == 2) {
+          if (a.data_type == 1) { if (nv > a.thr) nv = a.thr; }
+          else { if (nv < a.thr) nv = a.thr; }
+        }
+        a.m[row0 + Yj[j]] = nv;
+        mv = nv;
+      }
+      v[r].x = mv / cj[j];
+    } else {
+      v[r].x = rj[j].x / cj[j];
+      v[r].y = rj[j].y / cj[j];
+    }
+  }
+}
+
 template <int NR, int RPB>
 __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
   constexpr int T = NR / 16;
@@ -58,43 +115,17 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
   const int s = valid ? row / a.N0 : 0;
   const int X = valid ? row % a.N0 : 0;
   const int h1 = a.N1 >> 1;
-  FftTwiddles Tw;
-  load_twiddles<NR, -1>(Tw, t, a.tw);
   // the N1 data columns sit at y in [NR - N1/2, NR) and [0, N1 - N1/2), so
   // with N1 <= NR/2 the inputs 4..11 of every thread are padding
   float2 v[16];
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    v[r] = make_float2(0.f, 0.f);
-    if (r >= 4 && r < 12) continue;
-    const int y = in_index<NR>(t, r);
-    const int Y = (y + h1) & (NR - 1);
-    float2 gv = make_float2(0.f, 0.f);
-    if (valid && Y < a.N1) {
-      const size_t idx = ((size_t)s * a.N0 + X) * a.N1 + Y;
-      const float c = a.ck[(size_t)X * a.N1 + Y];
-      if (!a.cplx) {
-        float mv = a.m[idx];
-        if (a.update) {
-          // m - al*Re(rhat) in double, stored as float, then the threshold
-          const double al = a.alpha[s];
-          float nv = (float)__fma_rn(-al, (double)a.rhat[idx].x, (double)mv);
-          if (a.update == 2) {
-            if (a.data_type == 1) { if (nv > a.thr) nv = a.thr; }
-            else { if (nv < a.thr) nv = a.thr; }
-          }
-          a.m[idx] = nv;
-          mv = nv;
-        }
-        gv.x = mv / c;
-      } else {
-        const float2 rv = a.rhat[idx];
-        gv.x = rv.x / c;
-        gv.y = rv.y / c;
-      }
-    }
-    v[r] = gv;
+  if (!a.cplx) {
+    if (a.update) row_fwd_inputs<NR, false, true>(v, a, t, s, X, valid, h1);
+    else row_fwd_inputs<NR, false, false>(v, a, t, s, X, valid, h1);
+  } else {
+    row_fwd_inputs<NR, true, false>(v, a, t, s, X, valid, h1);
   }
+  FftTwiddles Tw;
+  load_twiddles<NR, -1>(Tw, t, a.tw);
   fft_regs_half<NR, RowLay, -1>(v, lds + team * kRowLds<NR>, t, Tw);
   if (valid) {
     float2* o = a.out + (size_t)row * NR;
@@ -250,18 +281,32 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
   // its output in the LDS tile for the interpolation
   {
     const int c = tid % C, t = tid / C;
+    float2 v[16];
+    // rows x in [NC/4, 3NC/4) (inputs 4..11) are padding, as N0 <= NC/2;
+    // the other inputs are loaded unconditionally (rows past N0, only when
+    // N0 < NC/2, read row 0 and are dropped afterwards).  A conditional load
+    // per input would be compiled into branches whose merges wait for each
+    // load before the next is issued; the empty asm below takes the loaded
+    // values unconditionally, so that all loads are issued first.
+    float2 g[8];
+    bool ok[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int X = (in_index<NC>(t, j < 4 ? j : j + 8) + h0) & (NC - 1);
+      ok[j] = X < a.N0;
+      g[j] = a.in[((size_t)s * a.N0 + (ok[j] ? X : 0)) * a.n1 + y0 + c];
+    }
+    // the twiddles (first used after the first pass) are requested after
+    // the grid rows, so that the first pass waits for the rows only
     FftTwiddles T;
     load_twiddles<NC, -1>(T, t, a.tw);
-    float2 v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      // rows x in [NC/4, 3NC/4) (inputs 4..11) are padding, as N0 <= NC/2
-      v[r] = make_float2(0.f, 0.f);
-      if (r >= 4 && r < 12) continue;
-      const int x = in_index<NC>(t, r);
-      const int X = (x + h0) & (NC - 1);
-      if (X < a.N0) v[r] = a.in[((size_t)s * a.N0 + X) * a.n1 + y0 + c];
-    }
+    for (int j = 0; j < 8; j++) asm volatile("" : "+v"(g[j].x), "+v"(g[j].y));
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (ok[j]) v[j < 4 ? j : j + 8] = g[j];
     fft_regs_half<NC, ColFftLayFwd<C>, -1>(v, tile + c, t, T);
     // the last exchange ended with a barrier: the column can be overwritten
 #pragma unroll
