@@ -139,6 +139,56 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
 // image): rhat / ck padded to NR and y-transformed, written to a.fwd_out.
 // The inverse's output slot i and the forward's input slot r of a thread hold
 // the same element t + k T, so the hand-over is a fixed register permutation.
+// Output slots of the inverse row transform that the crop keeps: the image
+// (MODE 0: A^H result), or rhat = image - ahwb with its |rhat|^2 added to acc
+// and, FWD, rhat / ck placed in u as the forward transform's input.  The ck
+// and ahwb values of all slots are loaded before any is used (clamped to
+// column 0 where the slot is padding, the value then dropped): a load under
+// a per-lane condition would be compiled into a branch whose merge waits for
+// it before the next load is issued.
+template <int NR, bool MODE1, bool FWD>
+__device__ __forceinline__ void row_adj_outputs(const float2 (&v)[16], float2 (&u)[16], double& acc,
+                                                const RowAdjArgs& a, int t, int s, int X, bool valid, int h1) {
+  const size_t row0 = ((size_t)s * a.N0 + X) * a.N1;
+  float cj[16];
+  float2 wj[16];
+  bool ok[16];
+  int Yj[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (out_slot_cropped<NR>(i)) continue;   // padding columns, as N1 <= NR/2
+    const int Y = (out_index<NR>(t, i) + h1) & (NR - 1);
+    ok[i] = valid && Y < a.N1;
+    Yj[i] = ok[i] ? Y : 0;
+    cj[i] = a.ck[(size_t)X * a.N1 + Yj[i]];
+    if constexpr (MODE1) wj[i] = a.ahwb[row0 + Yj[i]];
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (out_slot_cropped<NR>(i)) continue;
+    asm volatile("" : "+v"(cj[i]));
+    if constexpr (MODE1) asm volatile("" : "+v"(wj[i].x), "+v"(wj[i].y));
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (out_slot_cropped<NR>(i)) continue;
+    if (!ok[i]) continue;
+    const float c = cj[i];
+    const float2 f = make_float2(v[i].x / c, v[i].y / c);
+    if constexpr (!MODE1) {
+      a.img[row0 + Yj[i]] = f;
+    } else {
+      const float2 r = make_float2(f.x - wj[i].x, f.y - wj[i].y);
+      a.rhat[row0 + Yj[i]] = r;
+      acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+      if constexpr (FWD) {
+        constexpr int RL = last_radix<NR>();
+        u[i / RL + (i % RL) * (16 / RL)] = make_float2(r.x / c, r.y / c);
+      }
+    }
+  }
+}
+
 template <int NR, int RPB, bool FWD>
 __global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
   constexpr int T = NR / 16;
@@ -152,41 +202,20 @@ __global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
   const int X = valid ? row % a.N0 : 0;
   FftTwiddles Tw;
   load_twiddles<NR, 1>(Tw, t, a.tw);
+  // the row (row 0 for a team past the last row, its results dropped): all
+  // 16 loads unconditional, so that they are in flight together
   float2 v[16];
   const float2* src = a.in + (size_t)(valid ? row : 0) * NR;
 #pragma unroll
-  for (int r = 0; r < 16; r++) v[r] = valid ? src[in_index<NR>(t, r)] : make_float2(0.f, 0.f);
+  for (int r = 0; r < 16; r++) v[r] = src[in_index<NR>(t, r)];
   fft_regs<NR, RowLay, 1>(v, lds + team * kRowLds<NR>, t, Tw);
   const int h1 = a.N1 >> 1;
   double acc = 0.0;
   float2 u[16];   // FWD: forward input, slot r holds element t + r T
 #pragma unroll
   for (int r = 0; r < 16; r++) u[r] = make_float2(0.f, 0.f);
-  if (valid) {
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      if (out_slot_cropped<NR>(i)) continue;   // padding columns, as N1 <= NR/2
-      const int y = out_index<NR>(t, i);
-      const int Y = (y + h1) & (NR - 1);
-      if (Y < a.N1) {
-        const size_t idx = ((size_t)s * a.N0 + X) * a.N1 + Y;
-        const float c = a.ck[(size_t)X * a.N1 + Y];
-        const float2 f = make_float2(v[i].x / c, v[i].y / c);
-        if (a.mode == 0) {
-          a.img[idx] = f;
-        } else {
-          const float2 w = a.ahwb[idx];
-          const float2 r = make_float2(f.x - w.x, f.y - w.y);
-          a.rhat[idx] = r;
-          acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-          if constexpr (FWD) {
-            constexpr int RL = last_radix<NR>();
-            u[i / RL + (i % RL) * (16 / RL)] = make_float2(r.x / c, r.y / c);
-          }
-        }
-      }
-    }
-  }
+  if (a.mode == 0) row_adj_outputs<NR, false, false>(v, u, acc, a, t, s, X, valid, h1);
+  else row_adj_outputs<NR, true, FWD>(v, u, acc, a, t, s, X, valid, h1);
   if (a.mode == 1) {
     red[threadIdx.x] = acc;
     __syncthreads();
