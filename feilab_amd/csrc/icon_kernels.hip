@@ -439,14 +439,16 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   const float2* fw = a.fw + (size_t)s * a.E;
   float* stage = reinterpret_cast<float*>(tile);
   const int x0 = tid * RL;
+  // range 0: the entries whose taps wrap from the top rows (threads of the
+  // first K-1 rows only; empty, [xo[NC], xo[NC]), for the others); range 1:
+  // the rest.  The four loads are unconditional, so that they are in flight
+  // together.
+  const bool wrap = x0 < K - 1;
   int lo[2], hi[2];
-  if (x0 >= K - 1) {
-    lo[0] = hi[0] = 0;
-    lo[1] = xo[x0 - (K - 1)]; hi[1] = xo[x0 + RL];
-  } else {
-    lo[0] = xo[NC - (K - 1) + x0]; hi[0] = xo[NC];
-    lo[1] = xo[0]; hi[1] = xo[x0 + RL];
-  }
+  lo[0] = xo[wrap ? NC - (K - 1) + x0 : NC];
+  hi[0] = xo[NC];
+  lo[1] = xo[wrap ? 0 : x0 - (K - 1)];
+  hi[1] = xo[x0 + RL];
   float2 acc[RL][C];
 #pragma unroll
   for (int r = 0; r < RL; r++) {
