@@ -165,16 +165,36 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
       if (q < kInQ) *reinterpret_cast<uint4*>(&s_in[(q >> 2) * kLdsRow + 8 * (q & 3)]) = pin[i];
     }
     if (with_w) {
+      // in groups of 3: the group's loads are unconditional (piece 0 where
+      // the piece is padding, the value then dropped) and taken by an empty
+      // asm before any is stored, so that they are in flight together (a
+      // conditional load would be waited for before the next one is issued)
       const int ci0 = 32 * ck;
 #pragma unroll
-      for (int i = 0; i < kWPer; i++) {
-        const int q = tid + 256 * i;
-        const int c = q & 3, r = q >> 2;
-        const int co = co0 + r % COB, t = r / COB, ci = ci0 + 8 * c;
-        const bool ok = q < wq && co < a.Cout && ci < a.Cin;
-        if (q < wq)
-          *reinterpret_cast<uint4*>(&s_w[r * kLdsRow + 8 * c]) =
-              ok ? *reinterpret_cast<const uint4*>(a.w + ((long)t * a.Cout + co) * a.Cin + ci) : make_uint4(0, 0, 0, 0);
+      for (int i0 = 0; i0 < kWPer; i0 += 3) {
+        uint4 wv[3];
+        bool okv[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (i0 + k >= kWPer) break;
+          const int q = tid + 256 * (i0 + k);
+          const int c = q & 3, r = q >> 2;
+          const int co = co0 + r % COB, t = r / COB, ci = ci0 + 8 * c;
+          okv[k] = q < wq && co < a.Cout && ci < a.Cin;
+          wv[k] = *reinterpret_cast<const uint4*>(a.w + (okv[k] ? ((long)t * a.Cout + co) * a.Cin + ci : 0));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (i0 + k >= kWPer) break;
+          asm volatile("" : "+v"(wv[k].x), "+v"(wv[k].y), "+v"(wv[k].z), "+v"(wv[k].w));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (i0 + k >= kWPer) break;
+          const int q = tid + 256 * (i0 + k);
+          if (q < wq)
+            *reinterpret_cast<uint4*>(&s_w[(q >> 2) * kLdsRow + 8 * (q & 3)]) = okv[k] ? wv[k] : make_uint4(0, 0, 0, 0);
+        }
       }
     }
   };
@@ -559,12 +579,23 @@ __global__ __launch_bounds__(256) void k_conv_first(const uint16_t* x, int ldx, 
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
     const int px = (int)(p % W), py = (int)((p / W) % H);
     const long pn = p / ((long)W * H);
-    float v[9];
+    // the 9 taps are loaded unconditionally (the pixel itself where a tap
+    // falls outside the image, the value then dropped) and taken by an empty
+    // asm before any is used: a load under a per-lane condition would be
+    // compiled into a branch whose merge waits for it before the next load
+    uint32_t raw[9];
+    bool ok[9];
 #pragma unroll
     for (int t = 0; t < 9; t++) {
       const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-      v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? bf2f(x[((pn * H + yy) * W + xx) * ldx]) : 0.f;
+      ok[t] = yy >= 0 && yy < H && xx >= 0 && xx < W;
+      raw[t] = x[((pn * H + (ok[t] ? yy : py)) * W + (ok[t] ? xx : px)) * ldx];
     }
+#pragma unroll
+    for (int t = 0; t < 9; t++) asm volatile("" : "+v"(raw[t]));
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) v[t] = ok[t] ? bf2f((uint16_t)raw[t]) : 0.f;
     for (int c = 0; c < Cout; c += 8) {
       V8 o;
 #pragma unroll
