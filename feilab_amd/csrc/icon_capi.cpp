@@ -354,6 +354,17 @@ int icon_device_compute_units(int device, int* cus) {
   return ICON_OK;
 }
 
+int icon_set_device(int device) {
+  HIPCHK(hipSetDevice(device));
+  return ICON_OK;
+}
+
+int icon_get_device(int* device) {
+  if (!device) return fail(ICON_ERR_ARG, "icon_get_device: null pointer");
+  HIPCHK(hipGetDevice(device));
+  return ICON_OK;
+}
+
 int icon_malloc(void** dptr, size_t bytes) {
   if (!dptr) return fail(ICON_ERR_ARG, "icon_malloc: null pointer");
   HIPCHK(hipMalloc(dptr, bytes ? bytes : 1));

@@ -96,18 +96,24 @@ extern "C" const char* icon_lib_2d2_error(void) {
 }
 
 // Slice partition across devices (ICONGPU.cu:399-453): shares proportional to
-// the compute-unit count (floored), leftovers handed out round-robin starting
-// with the device of most units; the last device takes whatever remains.
+// the compute-unit count (floored), leftovers handed out round-robin in the
+// order of most units first; the last device takes whatever remains.  Ties in
+// that order resolve as the reference's selection loop does (ICONGPU.cu:
+// 402-414): each pick starts from the last device not yet ordered and moves
+// only to a device of strictly more units, scanning from device 0, so equal
+// devices are ordered last index first.
 extern "C" int icon_partition_slices(int ndev, const int* cus, int y_start, int y_end, int* first, int* last) {
   const int total = y_end - y_start + 1;
   if (ndev < 1 || total < 1) return ICON_ERR_ARG;
   std::vector<int> rate(ndev, 0), order(ndev, 0), taken(ndev, 0);
   for (int d = 0; d < ndev; d++) {
-    int best = -1;
-    for (int i = 0; i < ndev; i++)
-      if (!taken[i] && (best < 0 || cus[best] < cus[i])) best = i;
-    order[d] = best;
-    taken[best] = 1;
+    int pick = -1;
+    for (int i = ndev - 1; i >= 0 && pick < 0; i--)
+      if (!taken[i]) pick = i;
+    for (int j = 0; j < ndev; j++)
+      if (!taken[j] && cus[pick] < cus[j]) pick = j;
+    order[d] = pick;
+    taken[pick] = 1;
   }
   double sum = 0;
   for (int d = 0; d < ndev; d++) sum += cus[d];
@@ -198,10 +204,21 @@ static int lib_2d(bool cv, int device, int slice_begin, int slice_end, int itern
     set_err(icon_last_error());
     return rc;
   }
+  // The solve of each batch is replayed as a HIP graph on device buffers
+  // allocated once for the whole slice range (the last, shorter batch
+  // captures its own graph).
   icon_solve_opts o{};
-  o.pre = pre; o.icon = icon; o.post = post; o.data_type = data_type; o.threshold = thr; o.use_graph = 0;
+  o.pre = pre; o.icon = icon; o.post = post; o.data_type = data_type; o.threshold = thr; o.use_graph = 1;
   std::vector<float> sino((size_t)B * nz * nx), m((size_t)B * nx * thickness), rp((size_t)B * nx);
-  for (int y0 = slice_begin; y0 < slice_end; y0 += B) {
+  void *d_sino = nullptr, *d_m = nullptr, *d_rp = nullptr;
+  int prev_dev = -1;
+  if (icon_get_device(&prev_dev) != ICON_OK) prev_dev = -1;
+  if (icon_set_device(device) != ICON_OK || icon_malloc(&d_sino, sizeof(float) * sino.size()) != ICON_OK ||
+      icon_malloc(&d_m, sizeof(float) * m.size()) != ICON_OK || icon_malloc(&d_rp, sizeof(float) * rp.size()) != ICON_OK) {
+    rc = ICON_ERR_DEVICE;
+    set_err(icon_last_error());
+  }
+  for (int y0 = slice_begin; rc == ICON_OK && y0 < slice_end; y0 += B) {
     const int cnt = (slice_end - y0) < B ? (slice_end - y0) : B;
     if (!mrc::read_y_slices(fin, head, y0, cnt, sino.data(), &err)) {
       rc = ICON_ERR_IO;
@@ -209,7 +226,11 @@ static int lib_2d(bool cv, int device, int slice_begin, int slice_end, int itern
       break;
     }
     auto t0 = std::chrono::steady_clock::now();
-    rc = icon_reconstruct_host(plan, cnt, sino.data(), m.data(), nullptr, rp.data(), nullptr, nullptr, &o);
+    rc = icon_memcpy_h2d(d_sino, sino.data(), sizeof(float) * (size_t)cnt * nz * nx);
+    if (!rc) rc = icon_reconstruct(plan, cnt, d_sino, d_m, nullptr, d_rp, nullptr, nullptr, &o, nullptr);
+    if (!rc) rc = icon_device_synchronize();
+    if (!rc) rc = icon_memcpy_d2h(m.data(), d_m, sizeof(float) * (size_t)cnt * nx * thickness);
+    if (!rc) rc = icon_memcpy_d2h(rp.data(), d_rp, sizeof(float) * (size_t)cnt * nx);
     if (rc) {
       set_err(icon_last_error());
       break;
@@ -234,7 +255,11 @@ static int lib_2d(bool cv, int device, int slice_begin, int slice_end, int itern
     if (rc) break;
   }
   if (!cv) std::printf("full reconstruction\n");
+  icon_free(d_sino);
+  icon_free(d_m);
+  icon_free(d_rp);
   icon_plan_destroy(plan);
+  if (prev_dev >= 0) icon_set_device(prev_dev);
   std::fclose(fin);
   return rc;
 }

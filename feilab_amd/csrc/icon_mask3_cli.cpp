@@ -8,28 +8,25 @@
 #include "../../include/icon_mi355x.h"
 
 static void help() {
-  std::printf("ICONMask3 parameter\n######\n");
-  std::printf("-inputPath (-i) : the folder that contains all 2D reconstructed slices (named midxxxxx.mrc), "
-              "normally corresponding to the reconstruction folder generated by ICON.\n######\n");
-  std::printf("-tiltfile (-t) : the aligned tilt file.\n######\n");
-  std::printf("-output (-o) : the masked 3D reconstruction.\n######\n");
-  std::printf("-slice (-s) : the reconstructed slices for combination including 2 parts split by ','. For example, "
-              "0,511 means that combining 512 slices ranging from slice 0 (mid00000.mrc) to slice 511 "
-              "(mid00511.mrc).\n######\n");
-  std::printf("-thikness (-th) : the thickness of the final masked 3D reconstruction in pixel.\n######\n");
-  std::printf("-radius (-r) : the mask radius (in pixel) used in the Fourier domain of the combined 3D "
-              "reconstruction. If this option is used, 'crossVfrc' and 'fullRecfrc' are not used.\n######\n");
-  std::printf("-gaussWidth (-gw) : the gaussian width of the soft mask (in pixel). If '-crossVfrc' & '-fullRecVfrc' "
-              "are used, the default value is calculated accroding to FRC0.3 - FRC0.5; if '-radius' is used, the "
-              "default value is 10.\n######\n");
-  std::printf("-crossVfrc (-cf) : the FRC curve from the cross validation process. If 'radius' is used, this option "
-              "is not used.\n######\n");
-  std::printf("-fullRecfrc (-ff) : the FRC file from the full reconstruction process. If 'radius' is used, this "
-              "option is not used.\n######\n");
-  std::printf("-zshift (-z) : the shift (in pixel) of sample in Z axis, default as 0.\n#####\n");
-  std::printf("-nomask (-nm) : if this value is set to 0 then a validation filtering mask will be executed; "
-              "otherwise, no validation filtering mask will be executed, default as 0.\n######\n");
-  std::printf("-help (-h) : for help.\n");
+  std::printf(
+      "usage: icon-mask3 -i DIR -s FIRST,LAST -th ROWS -t TILTS -o OUT.mrc\n"
+      "                  [-r PX | -cf CROSSV.frc -ff FULLREC.frc] [-gw PX] [-z PX] [-nm 0|1]\n"
+      "Stacks the slices DIR/midYYYYY.mrc (ICON's reconstruction folder) into one volume,\n"
+      "each slice filtered in 2D Fourier space by a mask built from the tilt range and\n"
+      "the resolution the cross-validation FRC supports.\n"
+      "  -inputPath, -i   folder holding the reconstructed slices midYYYYY.mrc\n"
+      "  -slice, -s       first and last slice to stack, inclusive (\"0,511\": 512 slices)\n"
+      "  -thickness, -th  rows kept around the slice centre (the output's depth, pixels)\n"
+      "  -tiltfile, -t    tilt angles of the series (degrees, as aligned)\n"
+      "  -output, -o      the stacked, filtered volume (MRC)\n"
+      "  -radius, -r      Fourier mask radius in pixels; replaces the two FRC files\n"
+      "  -crossVfrc, -cf  FRC of the cross-validation reprojection (ICON's crossV.frc)\n"
+      "  -fullRecfrc, -ff FRC of the full reconstruction's reprojection (fullRec.frc)\n"
+      "  -gaussWidth, -gw width in pixels of the mask's Gaussian edge (default: the gap\n"
+      "                   between the FRC-0.5 and FRC-0.3 radii, or 10 with -radius)\n"
+      "  -zshift, -z      depth offset of the kept rows in pixels (default 0)\n"
+      "  -noMask, -nm     1: stack the cropped slices without filtering (default 0)\n"
+      "  -help, -h        this text\n");
 }
 
 int main(int argc, char** argv) {
@@ -61,14 +58,14 @@ int main(int argc, char** argv) {
   if (m[9]) std::printf("gaussWidth : %d\n", gw);
   std::printf("noMask : %d\n", no_mask);
   if (!(m[0] && m[1] && m[2] && m[4] && m[5])) {
-    std::printf("parameter error!\n'-inputPath' or '-slice' or '-thickness' or '-tiltfile' or '-output' are "
-                "missed!\nPlease use -help to see the manual\n");
+    std::printf("parameter error!\nneeded: -inputPath, -slice, -thickness, -tiltfile and -output "
+                "(-help lists the options)\n");
     return -1;
   }
-  if (no_mask == 1) std::printf("'-noMask' is 1; simple Catch is executed without validation filtering mask!\n");
+  if (no_mask == 1) std::printf("-noMask 1: slices are cropped and stacked without the Fourier mask\n");
   if (no_mask == 0 && !m[3] && !(m[6] && m[7])) {
-    std::printf("parameter error!\n'-radius' or ('-crossVfrc' & 'fullRecfrc') are missed!\nPlease use -help to see "
-                "the manual\n");
+    std::printf("parameter error!\nthe mask needs -radius, or both -crossVfrc and -fullRecfrc "
+                "(-help lists the options)\n");
     return -1;
   }
   const int rc = icon_mask3(0, in.c_str(), s0, s1, thickness, tilt.c_str(), out.c_str(), m[3] ? radius : 0,

@@ -1,6 +1,6 @@
 // icon-preprocess: the ICONPreProcess command line
 // (sourceCode/ICONPreProcess/ICONPreProcess.c:17-25,90-215): the same flags,
-// messages and outputs; the work runs on device 0.
+// defaults, exit codes and outputs; the work runs on device 0.
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -8,14 +8,18 @@
 #include "../../include/icon_mi355x.h"
 
 static void help() {
-  std::printf("preProcess parameter\n######\n");
-  std::printf("-input (-i) : the tilt series.\n######\n");
-  std::printf("-tiltfile (-t) : the file containing aligned tilt angle of each projection image. If this option is "
-              "not used, then only subtract the mode value of projection images.\n######\n");
-  std::printf("-thickness (-th) : the thickness of specimen in pixel. If this option is not used, then only subtract "
-              "the mode value of projection images\n######\n");
-  std::printf("-output (-o) : preProcessed projection file.\n######\n");
-  std::printf("-help (-h) : for help.\n");
+  std::printf(
+      "usage: icon-preprocess -i IN.mrc -o OUT.mrc [-t TILTS -th THICKNESS]\n"
+      "Subtracts from every projection the mode of its central region's histogram and,\n"
+      "given the tilt angles and the specimen thickness, scales each projection to the\n"
+      "variance 0.33 * thickness / cos(tilt).\n"
+      "  -input, -i      tilt series to prepare (MRC)\n"
+      "  -output, -o     prepared tilt series (MRC, float)\n"
+      "  -tiltfile, -t   aligned tilt angle of every projection (degrees); without it\n"
+      "                  only the mode is subtracted\n"
+      "  -thickness, -th specimen thickness in pixels; without it only the mode is\n"
+      "                  subtracted\n"
+      "  -help, -h       this text\n");
 }
 
 int main(int argc, char** argv) {
@@ -41,10 +45,10 @@ int main(int argc, char** argv) {
     }
   }
   if (!(has_in && has_out)) {
-    std::printf("parameter error!\n input or output is missed.\nPlease use -help to see the manual\n");
+    std::printf("parameter error!\nneeded: -input and -output (-help lists the options)\n");
     return 0;
   }
-  if (!(has_tilt && has_th)) std::printf("tiltfile or thickness is not specified. Normalization will not be executed\n");
+  if (!(has_tilt && has_th)) std::printf("no tilt file or thickness: the mode is subtracted, the variance is left as is\n");
   std::printf("parameter:\n");
   std::printf("input : %s\n", in.c_str());
   if (has_tilt) std::printf("tiltfile : %s\n", tilt.c_str());

@@ -87,6 +87,9 @@ typedef struct {
 const char* icon_last_error(void);
 const char* icon_version(void);
 int icon_device_count(int* count);
+/* The calling thread's current device (thin HIP wrappers). */
+int icon_set_device(int device);
+int icon_get_device(int* device);
 /* Device memory helpers on the current device (thin HIP wrappers). */
 int icon_malloc(void** dptr, size_t bytes);
 int icon_free(void* dptr);

@@ -160,6 +160,49 @@ def test_bench_geometry_solve_vs_oracle(icon, oracle_mod):
         assert abs(res[k][-1] - reso[-1]) / reso[-1] < 1e-4
 
 
+def test_data_residual_vs_exact_transform(icon):
+    """The device's R(m) (FP64 in k_combine) against numpy alone: the exact
+    non-uniform DFT of the device's own iterate, the centred DFT of the
+    projection rows and the density weights; R(0) = 1 exactly; on data that
+    an image m* explains exactly R(m*) is the window error alone."""
+    from exact_ref import centred_dft_rows, consistent_case, ndft_np, nodes_np
+    _print("data residual vs exact transform")
+    N = 64
+    rng = np.random.default_rng(5)
+    tilt = phantom.tilt_series(60, 6)
+    sino = phantom.add_noise(phantom.project_2d(phantom.blobs_2d(N, 8, rng), N, tilt), 1.0, rng)
+    plan = icon.Plan(N, tilt, max_slices=1)
+    from feilab_amd.icon import geometry_tables
+    w = geometry_tables(N, tilt)["w"].astype(np.float64)
+    b = centred_dft_rows(sino)
+    xy = nodes_np(N, tilt)
+    for k in (0, 1, 4):
+        m, res, _ = plan.solve(sino, k, 0, 0, 2, 0.0, trace=True)
+        assert res[0] == 1.0
+        R = np.sqrt(np.sum(w * np.abs(ndft_np(m, xy) - b) ** 2) / np.sum(w * np.abs(b) ** 2))
+        _print("k", k, "device R", res[-1], "exact", R)
+        assert abs(res[-1] - R) < 5e-6
+    ms, p = consistent_case(N, tilt)
+    _, res, _ = plan.solve(p, 0, 0, 0, 2, 0.0, m_init=ms, trace=True)
+    assert res[0] < 1e-5
+
+
+def test_icon_phase_bounds_on_device(icon):
+    """The clamp runs in iterations pre <= it < iters - post (Iteration2.cu:129),
+    bitwise: pre = 2, icon = 1 is the 3-step unclamped solve clamped once, and
+    a following free step (post = 1) equals a warm-started 1-step solve."""
+    tilt, s0 = _small_case(17, N=64, nb=8)
+    plan = icon.Plan(64, tilt, max_slices=1)
+    unclamped = plan.solve(s0, 3, 0, 0, 2, 0.0)
+    assert (unclamped < 0).sum() > 50
+    clamped = plan.solve(s0, 2, 1, 0, 2, 0.0)
+    np.testing.assert_array_equal(clamped, np.maximum(unclamped, 0.0))
+    np.testing.assert_array_equal(plan.solve(s0, 2, 1, 0, 1, 0.01), np.minimum(unclamped, np.float32(0.01)))
+    full = plan.solve(s0, 2, 1, 1, 2, 0.0)
+    np.testing.assert_array_equal(full, plan.solve(s0, 0, 0, 1, 2, 0.0, m_init=clamped))
+    assert (full < 0).sum() > 50
+
+
 def test_batch_matches_single(icon):
     """Slices of a batch are independent: solving them together gives each
     slice's single-slice result."""

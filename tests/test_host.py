@@ -145,6 +145,19 @@ def test_partition_follows_compute_units(icon):
     assert icon.partition_slices([1, 1, 1, 1], 5, 6) == [(5, 5), (6, 6)]
 
 
+def test_partition_ties_follow_the_reference_loop(icon):
+    """Leftover slices go out in the reference's device order (ICONGPU.cu:
+    402-414), worked by hand: each pick starts at the last device not yet
+    ordered and moves only to a strictly larger count, so 8 equal devices are
+    ordered 7, 6, ..., 0 and the two leftovers of 10 slices go to devices 7
+    and 6 (device 7, the last, takes the rest)."""
+    assert icon.partition_slices([1] * 8, 0, 9) == [(0, 0), (1, 1), (2, 2), (3, 3), (4, 4), (5, 5), (6, 7), (8, 9)]
+    # counts 1, 2, 2, 1 and 11 slices: floors 1, 3, 3, 1; the order is 1, 2, 3, 0,
+    # so the three leftovers go to devices 1, 2 and 3
+    assert icon.partition_slices([1, 2, 2, 1], 0, 10) == [(0, 0), (1, 4), (5, 8), (9, 10)]
+    assert icon.partition_slices([256, 256, 256, 128], 0, 499) == [(0, 142), (143, 285), (286, 428), (429, 499)]
+
+
 # ---------------------------------------------------------------- CLI
 
 def _cli():
@@ -198,7 +211,7 @@ def test_frc_smooth_window():
 def test_preprocess_cli_help_and_missing_flags():
     exe = os.path.join(ROOT, "feilab_amd", "icon-preprocess")
     r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
-    assert r.returncode == 0 and "-thickness (-th)" in r.stdout
+    assert r.returncode == 0 and "-thickness, -th" in r.stdout
     r = subprocess.run([exe, "-i", "x.mrc"], capture_output=True, text=True, timeout=30)
     assert "parameter error!" in r.stdout
 
@@ -219,10 +232,10 @@ def test_mask3_host_weights_match_oracle(icon):
 def test_mask3_cli_help_and_missing_flags():
     exe = os.path.join(ROOT, "feilab_amd", "icon-mask3")
     r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
-    assert r.returncode == 1 and "-crossVfrc (-cf)" in r.stdout
+    assert r.returncode == 1 and "-crossVfrc, -cf" in r.stdout
     r = subprocess.run([exe, "-i", "d", "-s", "0,3"], capture_output=True, text=True, timeout=30)
-    assert r.returncode != 0 and "are missed!" in r.stdout
+    assert r.returncode != 0 and "parameter error!" in r.stdout and "-tiltfile" in r.stdout
     # a mask needs '-radius' or both FRC files
     r = subprocess.run([exe, "-i", "d", "-s", "0,3", "-th", "8", "-t", "a.tlt", "-o", "o.mrc"],
                        capture_output=True, text=True, timeout=30)
-    assert r.returncode != 0 and "'-radius' or ('-crossVfrc' & 'fullRecfrc') are missed!" in r.stdout
+    assert r.returncode != 0 and "needs -radius, or both -crossVfrc and -fullRecfrc" in r.stdout
