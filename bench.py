@@ -6,20 +6,29 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
 41 tilts +-60 deg in 3 deg steps, SNR 0.05), sinograms resident in HBM.
 
 * Plain leg: one step = the batched plain ICON solve (200 iterations split
-  20 INFR / 160 ICON / 20 INFR, threshold m >= 0) replayed as one HIP graph.
-  Its per-slice data residual R(m) = sqrt(sum w|Am-b|^2 / sum w|b|^2) (FP64)
-  is the target, its reprojection FRC at the min-|theta| tilt the reference.
+  20 INFR / 160 ICON / 20 INFR, threshold m >= 0) replayed as one HIP graph,
+  with its per-slice data residual R(m) = sqrt(sum w|Am-b|^2 / sum w|b|^2)
+  (FP64) and reprojection, the residuals read back.  Each slice's final R is
+  the target.
 * Learned leg: a U-Net trained on the device (its own BF16 MFMA kernels) on
-  (4-iteration, 200-iteration) pairs of a separate phantom; one step = 4
-  plain iterations, the U-Net warm start, and k descent iterations, with k the
-  smallest continuation that brings every slice to R <= its plain target with
-  the reprojection FRC (rings 1..N/4) within 0.005 of plain ICON's.  Slices
-  that miss the target are re-solved by plain ICON (counted as fallbacks).
+  (10-iteration, 200-iteration) pairs of 20480 slices of other seeded
+  phantoms; one step = 10 plain iterations, the U-Net warm start, k descent
+  iterations with residuals and reprojection, the residuals read back, and
+  plain ICON re-solving every slice whose R misses its target (the per-slice
+  fallback, timed as part of the step).  k is calibrated once on a held-out
+  phantom of the training distribution, never on the evaluated series: the
+  smallest k at which every calibration slice reaches its plain-ICON
+  residual with an FRC against the phantom no more than 0.005 below plain
+  ICON's.
+* Quality (not timed): per slice, relative L2 and FRC of the learned result
+  against plain ICON's and against the phantom, and the same for the 10 + k
+  iterations without the network.
 
 Prints one JSON line: value = learned-start slices/s, plus the plain leg,
 the roofline of the dominant ICON kernel (algorithmic bytes / HIP-event-timed
 launch duration against 8 TB/s HBM) and the CPU baseline (the FP32-mirror
-oracle on the host cores on a bounded sample of the plain workload).
+oracle on the host cores, a fixed sample of the same workload solved in
+full).
 """
 import argparse
 import json
@@ -75,38 +84,62 @@ def algorithmic_bytes(plan):
 LAUNCHES_PER_ITER = {"row_fwd": 1, "col_fwd": 2, "combine": 2, "col_adj": 1, "row_adj": 1}
 
 
-def cpu_baseline(cfg, S_cpu, iters_cpu, total_iters, threads):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, sched, threads):
+    """The FP32-mirror oracle (OpenMP, one slice per thread) on `threads` host
+    threads: `threads` slices of the 512^2 series solved in full (200
+    iterations, no extrapolation) -> slices/s in the metric's unit; and all
+    of the 256^2 config (one slice, 61 tilts, 100 iterations) timed beside it."""
     import oracle
+    pre, icon_it, post = sched
     plan = oracle.Plan(cfg["N"], cfg["N"], cfg["tilt"], 32)
-    sinos = cfg["sino"][:S_cpu]
-    pre = min(cfg["pre"], iters_cpu)
+    S_cpu = threads
     t0 = time.perf_counter()
-    _, used = plan.solve_batch(sinos, pre, iters_cpu - pre, 0, 2, 0.0, threads=threads)
+    _, used = plan.solve_batch(cfg["sino"][:S_cpu], pre, icon_it, post, 2, 0.0, threads=threads)
     dt = time.perf_counter() - t0
-    value = S_cpu * (iters_cpu / total_iters) / dt
-    return dict(value=round(value, 4), unit="slices/s", cores=int(used), kind="port",
-                sample="%d slices x %d of %d iterations of the same 512^2 x %d-tilt series, FP32-mirror "
-                       "oracle (OpenMP over slices); slices/s scaled by %d/%d; %.1f s wall"
-                       % (S_cpu, iters_cpu, total_iters, len(cfg["tilt"]), iters_cpu, total_iters, dt))
+    from feilab_amd import phantom
+    c1 = phantom.config1()
+    p1 = oracle.Plan(256, 256, c1["tilt"], 32)
+    t1 = time.perf_counter()
+    p1.solve(c1["sino"], c1["pre"], c1["icon"], c1["post"], 2, 0.0)
+    dt1 = time.perf_counter() - t1
+    return dict(value=round(S_cpu / dt, 4), unit="slices/s", cores=int(used), kind="port",
+                cpu_model=cpu_model(), host_cpus=os.cpu_count(),
+                sample="%d slices of the same 512^2 x %d-tilt series, all %d iterations (%d,%d,%d), FP32-mirror "
+                       "oracle, OpenMP one slice per thread on %d threads; %.1f s wall"
+                       % (S_cpu, len(cfg["tilt"]), pre + icon_it + post, pre, icon_it, post, used, dt),
+                config1_seconds=round(dt1, 2),
+                config1="one 256^2 slice, 61 tilts, 100 iterations (10,80,10), one thread")
 
 
 COPY_BYTES = 1 << 30
 
 
 def iteration_roofline(plan, S, per_iter_ms):
-    """One ICON iteration against HBM with the reference's dataflow bytes per
-    slice, 17 G + 6 S + 8 I_c + 3 I_r (G = 8 n0 n1 grid, S = 8 M samples,
-    I_c = 8 N0 N1 complex image, I_r = 4 N0 N1 real image: three FFTs, pads,
-    spread, two interpolations, weights, residual and update as separate
-    passes).  Fusion moves fewer bytes, so the fraction can exceed 1."""
+    """One ICON iteration (7 launches) against HBM with this build's own
+    algorithmic bytes per slice: the per-kernel bytes of algorithmic_bytes()
+    times the launches per iteration.  The reference's dataflow (17 G + 6 S +
+    8 I_c + 3 I_r per slice: three FFTs, pads, spread, two interpolations,
+    weights, residual and update as separate passes) is reported beside it
+    as an equivalent rate, not as a fraction of the peak."""
+    kb = algorithmic_bytes(plan)
+    own = sum(kb[k] * n for k, n in LAUNCHES_PER_ITER.items())
+    ach = own * S / (per_iter_ms * 1e-3) / 1e9
     G = 8 * plan.n0 * plan.n1
-    Sb = 8 * plan.M
-    Ic = 8 * plan.N0 * plan.N1
-    Ir = 4 * plan.N0 * plan.N1
-    per_slice = 17 * G + 6 * Sb + 8 * Ic + 3 * Ir
-    ach = per_slice * S / (per_iter_ms * 1e-3) / 1e9
-    return dict(bytes_per_slice=per_slice, ms_per_iteration=round(per_iter_ms, 4), achieved=round(ach, 1),
-                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4))
+    ref = 17 * G + 6 * 8 * plan.M + 8 * 8 * plan.N0 * plan.N1 + 3 * 4 * plan.N0 * plan.N1
+    return dict(bytes_per_slice=own, ms_per_iteration=round(per_iter_ms, 4), achieved=round(ach, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                reference_dataflow_bytes_per_slice=ref,
+                reference_dataflow_equivalent_GBs=round(ref * S / (per_iter_ms * 1e-3) / 1e9, 1))
 
 
 def probe(what, S):
@@ -139,10 +172,11 @@ def probe(what, S):
     plan.time_kernel(what, S, 2)
 
 
-def counter_pass(counter, target, S, kernel_substr):
-    """Average per-dispatch value (KiB) of one TCC counter for the dispatches
-    whose name contains kernel_substr, from a rocprofv3 --pmc child run of the
-    probe mode; None if the profiler is unavailable or fails."""
+def counter_pass(counters, target, S, kernel_substr):
+    """Average per-dispatch values of TCC counters (summed over all instances)
+    for the dispatches whose name contains kernel_substr, from one
+    rocprofv3 --pmc child run of the probe mode; None if the profiler is
+    unavailable or fails."""
     import csv
     import shutil
     import subprocess
@@ -151,42 +185,221 @@ def counter_pass(counter, target, S, kernel_substr):
     if not os.path.exists(exe):
         return None
     d = tempfile.mkdtemp(prefix="bench_pmc_")
-    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--",
-           sys.executable, os.path.abspath(__file__), "--probe", target, "--slices", str(S)]
+    cmd = [exe, "--pmc"] + list(counters) + ["--output-format", "csv", "-d", d, "-o", "p", "--",
+                                             sys.executable, os.path.abspath(__file__), "--probe", target,
+                                             "--slices", str(S)]
     try:
         r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120,
                            env=dict(os.environ, TMPDIR=d))
     except subprocess.TimeoutExpired:
-        log("counter pass %s on %s timed out" % (counter, target))
+        log("counter pass %s on %s timed out" % (counters, target))
         return None
     vals = {}
     for root, _, files in os.walk(d):
         for f in files:
             if f.endswith("counter_collection.csv"):
                 for row in csv.DictReader(open(os.path.join(root, f))):
-                    if kernel_substr in row["Kernel_Name"] and row["Counter_Name"] == counter:
-                        vals[row["Dispatch_Id"]] = vals.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+                    if kernel_substr in row["Kernel_Name"] and row["Counter_Name"] in counters:
+                        key = (row["Counter_Name"], row["Dispatch_Id"])
+                        vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
     shutil.rmtree(d, ignore_errors=True)
-    if r.returncode != 0 and not vals:
-        log("counter pass %s on %s failed (rc %d)" % (counter, target, r.returncode))
+    if r.returncode not in (0, 139) and not vals:   # 139: the profiler's own exit after complete results
+        log("counter pass %s on %s failed (rc %d)" % (counters, target, r.returncode))
         return None
-    return float(np.mean(list(vals.values()))) if vals else None
+    out = {}
+    for c in counters:
+        v = [x for (n, _), x in vals.items() if n == c]
+        if not v:
+            return None
+        out[c] = float(np.mean(v))
+    return out
+
+
+READ_COUNTERS = ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_BUBBLE_sum")
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
 
 
 def measure_traffic(kernel, S):
-    """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE, each
-    scaled by what the same counter reads for a plain copy of known size."""
-    name = {"col_fwd": "k_col_fwd", "col_adj": "k_col_adj"}[kernel]
-    f_k = counter_pass("FETCH_SIZE", kernel, S, name)
-    w_k = counter_pass("WRITE_SIZE", kernel, S, name)
-    f_c = counter_pass("FETCH_SIZE", "copy", S, "k_copy_f4")
-    w_c = counter_pass("WRITE_SIZE", "copy", S, "k_copy_f4")
-    if None in (f_k, w_k, f_c, w_c) or f_c <= 0 or w_c <= 0:
+    """HBM bytes per launch of `kernel` from the L2-to-memory request counters
+    split by size: reads = 128 TCC_BUBBLE + u (RDREQ - 32B - BUBBLE) + 32 RDREQ_32B,
+    writes = 64 WRREQ_64B + 32 (WRREQ - WRREQ_64B), where u, the bytes of a
+    plain read request, is calibrated on a 1 GiB copy (its reads then equal
+    1 GiB; on gfx950 u = 128, a full L2 line) and the copy's write bytes are
+    checked against 1 GiB the same way."""
+    name = {"col_fwd": "k_col_fwd", "col_adj": "k_col_adj", "row_fwd": "k_row_fwd", "row_adj": "k_row_adj"}[kernel]
+    rk = counter_pass(READ_COUNTERS, kernel, S, name)
+    wk = counter_pass(WRITE_COUNTERS, kernel, S, name)
+    rc = counter_pass(READ_COUNTERS, "copy", S, "k_copy_f4")
+    wc = counter_pass(WRITE_COUNTERS, "copy", S, "k_copy_f4")
+    if None in (rk, wk, rc, wc):
         return None, None
-    fs, ws = COPY_BYTES / (f_c * 1024), COPY_BYTES / (w_c * 1024)
-    traffic = f_k * 1024 * fs + w_k * 1024 * ws
-    return traffic, dict(fetch_kib=round(f_k, 1), write_kib=round(w_k, 1), copy_fetch_kib=round(f_c, 1),
-                         copy_write_kib=round(w_c, 1), fetch_scale=round(fs, 4), write_scale=round(ws, 4))
+    plain_c = rc["TCC_EA0_RDREQ_sum"] - rc["TCC_EA0_RDREQ_32B_sum"] - rc["TCC_BUBBLE_sum"]
+    if plain_c <= 0:
+        return None, None
+    u = (COPY_BYTES - 128 * rc["TCC_BUBBLE_sum"] - 32 * rc["TCC_EA0_RDREQ_32B_sum"]) / plain_c
+    wbytes = lambda w: 64 * w["TCC_EA0_WRREQ_64B_sum"] + 32 * (w["TCC_EA0_WRREQ_sum"] - w["TCC_EA0_WRREQ_64B_sum"])
+    rbytes = lambda r: (128 * r["TCC_BUBBLE_sum"] + u * (r["TCC_EA0_RDREQ_sum"] - r["TCC_EA0_RDREQ_32B_sum"] -
+                                                        r["TCC_BUBBLE_sum"]) + 32 * r["TCC_EA0_RDREQ_32B_sum"])
+    rd, wr = rbytes(rk), wbytes(wk)
+    detail = dict(read_bytes=int(rd), write_bytes=int(wr), read_request_bytes=round(u, 2),
+                  copy_write_over_1GiB=round(wbytes(wc) / COPY_BYTES, 4),
+                  counters={k: round(v, 1) for k, v in list(rk.items()) + list(wk.items())},
+                  copy_counters={k: round(v, 1) for k, v in list(rc.items()) + list(wc.items())})
+    return rd + wr, detail
+
+
+def pair_chunks(total, chunk, seed0, workers):
+    """Sinograms of seeded training phantoms (config-2 series of `chunk`
+    slices, seeds seed0, seed0 + 1, ...), generated ahead of use by a pool of
+    fresh interpreter processes."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    from feilab_amd import phantom
+    jobs = [(min(chunk, total - k), seed0 + i) for i, k in enumerate(range(0, total, chunk))]
+    with ProcessPoolExecutor(max_workers=workers, mp_context=multiprocessing.get_context("spawn")) as ex:
+        yield from ex.map(phantom.config2_sino, jobs)
+
+
+def band_mean(curves, band):
+    return np.asarray(curves)[:, band[0]:band[1]].mean(axis=1)
+
+
+def rel_rows(a, b):
+    a = np.asarray(a, np.float32).reshape(len(a), -1)
+    b = np.asarray(b, np.float32).reshape(len(b), -1)
+    return np.sqrt(np.einsum("ij,ij->i", a - b, a - b, dtype=np.float64) / np.einsum("ij,ij->i", b, b,
+                                                                                        dtype=np.float64))
+
+
+def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, plain_value, timed):
+    """Trains the warm start, calibrates k on a held-out phantom, times the
+    learned solve with its fallback on the evaluated series and measures its
+    quality.  Returns (learned dict, unet_roofline dict, (value, ms_step) or None)."""
+    import feilab_amd
+    from feilab_amd import frc as F
+    from feilab_amd import phantom, warmstart
+    from feilab_amd.icon import DeviceArray, SolveOptions
+    L = feilab_amd.lib()
+    S = d_sino.shape[0]
+    N = plan.N0
+    band = (1, N // 4)
+    iters = sum(sched)
+    workers = max(1, min(8, (os.cpu_count() or 2) - 1))
+
+    # training pairs and the network
+    log("training pairs: %d slices of seeded phantoms, (%d, %d) iterations, %d generator processes"
+        % (args.train_pairs, args.start, iters, workers))
+    t0 = time.perf_counter()
+    x, t = warmstart.make_pairs(plan, pair_chunks(args.train_pairs, args.pair_chunk, 5001, workers),
+                                args.train_pairs, args.start, sched, log=log)
+    pairs_s = time.perf_counter() - t0
+    log("pairs ready in %.1f s" % pairs_s)
+    t0 = time.perf_counter()
+    params, losses = warmstart.train_on_pairs(x, t, steps=args.train_steps, log=log)
+    train_s = time.perf_counter() - t0
+    del x, t
+    solver = warmstart.LearnedSolver(plan, params, S, start=args.start, schedule=sched, batch=args.unet_batch)
+
+    # continuation length from a held-out phantom of the training distribution
+    Sc = args.calib_slices
+    cal = phantom.config2(nslices=Sc, seed=3001)
+    truth_c = phantom.render_slices_3d(cal["blobs"], N, N, np.arange(Sc))
+    d_c = DeviceArray.from_host(cal["sino"])
+    d_mc = DeviceArray((Sc, N, N), np.float32)
+    d_rc = DeviceArray((Sc, max(iters, args.max_cont) + 1), np.float64)
+    plan.reconstruct_dev(Sc, d_c, d_mc, SolveOptions(*sched, 2, 0.0, 1, 0), d_resid=d_rc)
+    L.icon_device_synchronize()
+    R_c = d_rc.numpy().reshape(-1)[:Sc * (iters + 1)].reshape(Sc, iters + 1)[:, -1]
+    q_plain_c = band_mean(F.frc_slices(d_mc.numpy(), truth_c), band)
+    k_star, scan = None, []
+    for k in range(1, args.max_cont + 1):
+        solver.run(Sc, d_c, d_mc, k, d_resid=d_rc)
+        L.icon_device_synchronize()
+        Rk = d_rc.numpy().reshape(-1)[:Sc * (k + 1)].reshape(Sc, k + 1)[:, -1]
+        qk = band_mean(F.frc_slices(d_mc.numpy(), truth_c), band)
+        row = dict(k=k, resid_max_over_target=round(float((Rk / R_c).max()), 4),
+                   frc_truth_drop_max=round(float((q_plain_c - qk).max()), 4),
+                   frc_truth_mean=round(float(qk.mean()), 4))
+        scan.append(row)
+        log("calibration k=%d:" % k, json.dumps(row))
+        if (Rk <= R_c).all() and (qk >= q_plain_c - args.frc_tol).all():
+            k_star = k
+            break
+    calib = dict(phantom_seed=3001, slices=Sc, frc_truth_plain_mean=round(float(q_plain_c.mean()), 4), scan=scan)
+    for a in (d_c, d_mc, d_rc):
+        a.free()
+    if k_star is None:
+        log("no continuation up to %d meets the calibration target: plain ICON is the result" % args.max_cont)
+        solver.close()
+        return dict(continuation_iterations=None, fallback="all slices (plain ICON)", calibration=calib,
+                    train_seconds=round(train_s, 1), pairs_seconds=round(pairs_s, 1)), None, None
+
+    # the timed learned solve on the evaluated series, fallback included
+    d_ml = DeviceArray((S, N, N), np.float32)
+    d_rl = DeviceArray((S, k_star + 1), np.float64)
+    d_rpl = DeviceArray((S, N), np.float32)
+    fallbacks = []
+
+    def learned_step():
+        fallbacks.append(len(solver.solve(S, d_sino, d_ml, k_star, R_plain, d_rl, d_rpl)))
+
+    dt_l = timed(learned_step, "learned")
+    value = S * args.steps / dt_l
+    ms_step = dt_l / args.steps * 1e3
+    R_l = d_rl.numpy()[:, -1]
+    m_l = d_ml.numpy()
+    _, frc_l = warmstart.reprojection_frc(d_rpl.numpy(), cfg["sino"], plan.iomit, band)
+
+    # quality: against plain ICON and against the phantom; the same iterations without the network
+    truth = phantom.render_slices_3d(cfg["blobs"], N, N, np.arange(S))
+    solver.run(S, d_sino, d_ml, k_star, d_resid=d_rl, d_reproj=d_rpl, use_net=False)
+    L.icon_device_synchronize()
+    m_n, R_n = d_ml.numpy(), d_rl.numpy()[:, -1]
+    _, frc_n = warmstart.reprojection_frc(d_rpl.numpy(), cfg["sino"], plan.iomit, band)
+    q_p = band_mean(F.frc_slices(m_plain, truth), band)
+    q_l = band_mean(F.frc_slices(m_l, truth), band)
+    q_n = band_mean(F.frc_slices(m_n, truth), band)
+    f_lp = band_mean(F.frc_slices(m_l, m_plain), band)
+    f_np = band_mean(F.frc_slices(m_n, m_plain), band)
+    e_lp, e_np = rel_rows(m_l, m_plain), rel_rows(m_n, m_plain)
+    e_pt, e_lt, e_nt = rel_rows(m_plain, truth), rel_rows(m_l, truth), rel_rows(m_n, truth)
+    st = lambda v: dict(mean=round(float(np.mean(v)), 4), min=round(float(np.min(v)), 4),
+                        max=round(float(np.max(v)), 4))
+    quality = dict(
+        frc_vs_phantom=dict(plain=st(q_p), learned=st(q_l), no_network=st(q_n),
+                            learned_minus_plain=st(q_l - q_p), no_network_minus_plain=st(q_n - q_p)),
+        rel_l2_vs_phantom=dict(plain=st(e_pt), learned=st(e_lt), no_network=st(e_nt)),
+        vs_plain_icon=dict(learned_rel_l2=st(e_lp), no_network_rel_l2=st(e_np), learned_frc=st(f_lp),
+                           no_network_frc=st(f_np)),
+        frc_band=list(band),
+        note="per slice; FRC = mean over rings %d..%d of calFRC's ring correlation of the two slices" % band)
+    learned = dict(start_iterations=args.start, continuation_iterations=k_star,
+                   slices_per_s=round(value, 3), speedup_vs_plain=round(value / plain_value, 3),
+                   resid_mean=float(R_l.mean()), resid_max_over_target=round(float((R_l / R_plain).max()), 5),
+                   fallback_slices_per_step=fallbacks[-1], fallback_slices_all_steps=fallbacks,
+                   reprojection_frc=round(frc_l, 5), reprojection_frc_plain=round(frc_plain, 5),
+                   no_network=dict(resid_max_over_target=round(float((R_n / R_plain).max()), 4),
+                                   reprojection_frc=round(frc_n, 5)),
+                   quality=quality, calibration=calib, train_pairs=args.train_pairs, train_steps=args.train_steps,
+                   train_loss_first=round(float(np.mean(losses[:20])), 4),
+                   train_loss_last=round(float(np.mean(losses[-50:])), 4),
+                   pairs_seconds=round(pairs_s, 1), train_seconds=round(train_s, 1), unet_batch=args.unet_batch)
+    log("learned:", json.dumps(learned))
+
+    # the U-Net forward alone
+    dt_u = timed(lambda: solver.net.warm_start_dev(S, solver.d_start, solver.d_init), "unet forward")
+    gf = unet_gflop_per_slice()
+    tf = gf * S * args.steps / dt_u / 1e3
+    unet_roofline = dict(bound="mfma", achieved=round(tf, 1), peak=BF16_DENSE_PEAK_TFLOPS, unit="TFLOP/s",
+                         frac=round(tf / BF16_DENSE_PEAK_TFLOPS, 4),
+                         kernel="U-Net forward, all layers (BF16 MFMA convolutions), batches of %d" % args.unet_batch,
+                         ms_per_256_slices=round(dt_u / args.steps * 1e3 * 256 / S, 2),
+                         gflop_per_slice=round(gf, 2))
+    log("unet forward:", json.dumps(unet_roofline))
+    for a in (d_ml, d_rl, d_rpl):
+        a.free()
+    solver.close()
+    return learned, unet_roofline, (value, ms_step)
 
 
 def main():
@@ -196,16 +409,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--iters", default="20,160,20", help="pre,icon,post")
-    ap.add_argument("--start", type=int, default=4, help="plain iterations before the U-Net")
+    ap.add_argument("--start", type=int, default=10, help="plain iterations before the U-Net")
     ap.add_argument("--train-steps", type=int, default=2000)
-    ap.add_argument("--train-slices", type=int, default=256)
+    ap.add_argument("--train-pairs", type=int, default=20480, help="(start, final) training pairs, slices")
+    ap.add_argument("--pair-chunk", type=int, default=256, help="slices per training phantom")
+    ap.add_argument("--calib-slices", type=int, default=64, help="slices of the held-out calibration phantom")
     ap.add_argument("--unet-batch", type=int, default=128, help="slices per U-Net forward launch sequence")
     ap.add_argument("--max-cont", type=int, default=40, help="continuation iterations tried")
-    ap.add_argument("--frc-tol", type=float, default=0.005)
+    ap.add_argument("--frc-tol", type=float, default=0.005,
+                    help="allowed drop of the FRC against the phantom below plain ICON's")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=16)
-    ap.add_argument("--cpu-iters", type=int, default=100)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads (and slices) of the CPU baseline; 0: OMP_NUM_THREADS or the CPU count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--col-block", type=int, default=0, help="grid columns per spread/interp tile (0: default)")
@@ -264,10 +479,15 @@ def main():
         log("%s: timed %d steps in %.3f s" % (what, args.steps, dt))
         return dt
 
-    # ---- plain leg
-    dt_plain = timed(lambda: plan.reconstruct_dev(S, d_sino, d_m, opts, d_resid=d_res, d_reproj=d_rp), "plain")
+    # ---- plain leg: the solve with residuals and reprojection, the residuals read back
+    def plain_step():
+        plan.reconstruct_dev(S, d_sino, d_m, opts, d_resid=d_res, d_reproj=d_rp)
+        return d_res.numpy()
+
+    dt_plain = timed(plain_step, "plain")
     plain_value = S * args.steps / dt_plain
     R_plain = d_res.numpy()[:, -1]
+    m_plain = d_m.numpy()
     frc_plain_curve, frc_plain = warmstart.reprojection_frc(d_rp.numpy(), cfg["sino"], iomit, band)
     log("plain: %.2f slices/s, R mean %.5f, FRC %.4f" % (plain_value, R_plain.mean(), frc_plain))
 
@@ -288,7 +508,7 @@ def main():
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
                     algo_bytes="%d per slice x %d slices per launch" % (kb[dom], S))
     if not args.no_traffic:
-        log("counter passes (FETCH_SIZE, WRITE_SIZE) on %s and a 1 GiB copy" % dom)
+        log("counter passes (L2-to-memory read and write requests by size) on %s and a 1 GiB copy" % dom)
         traffic, detail = measure_traffic(dom, S)
         if traffic is not None:
             roofline["traffic"] = int(traffic)
@@ -330,79 +550,23 @@ def main():
     unet_roofline = None
     value, ms_step = plain_value, dt_plain / args.steps * 1e3
     if not args.no_learned:
-        log("training the warm start: %d slices of a separate phantom, %d steps" % (args.train_slices, args.train_steps))
-        t0 = time.perf_counter()
-        tr = phantom.config2(nslices=args.train_slices, seed=2001)
-        params, losses = warmstart.train_warm_start(plan, tr["sino"], args.start, (pre, icon_it, post),
-                                                    steps=args.train_steps, log=log)
-        train_s = time.perf_counter() - t0
-        solver = warmstart.LearnedSolver(plan, params, S, start=args.start, batch=args.unet_batch)
-        d_ml = DeviceArray((S, plan.N0, plan.N1), np.float32)
-
-        def check(k, use_net):
-            d_r = DeviceArray((S, k + 1), np.float64)
-            solver.run(S, d_sino, d_ml, k, d_resid=d_r, d_reproj=d_rp, use_net=use_net)
-            L.icon_device_synchronize()
-            R = d_r.numpy()[:, -1]
-            _, f = warmstart.reprojection_frc(d_rp.numpy(), cfg["sino"], iomit, band)
-            ok = bool((R <= R_plain).all() and abs(f - frc_plain) <= args.frc_tol)
-            return ok, R, f
-
-        k_star, res = None, None
-        for k in range(1, args.max_cont + 1):
-            ok, R, f = check(k, True)
-            if ok:
-                k_star, res = k, (R, f)
-                break
-        nonet = []
-        for k in sorted({1, 5, 10, 20, args.max_cont} | ({k_star} if k_star else set())):
-            ok, R, f = check(k, False)
-            nonet.append(dict(cont=k, meets_target=ok, resid_max_over_target=round(float((R / R_plain).max()), 4),
-                              frc=round(f, 4)))
-        log("no-network continuation:", json.dumps(nonet))
-        if k_star is None:
-            log("learned start missed the target within %d continuation iterations: plain ICON is the result" % args.max_cont)
-            learned = dict(k=None, fallback="all slices (plain ICON)", train_seconds=round(train_s, 1), no_network=nonet)
-        else:
-            dt_l = timed(lambda: solver.run(S, d_sino, d_ml, k_star), "learned")
-            value = S * args.steps / dt_l
-            ms_step = dt_l / args.steps * 1e3
-            # verify the timed result and count slices that would fall back to plain ICON
-            d_r = DeviceArray((S, k_star + 1), np.float64)
-            solver.run(S, d_sino, d_ml, k_star, d_resid=d_r, d_reproj=d_rp)
-            L.icon_device_synchronize()
-            R = d_r.numpy()[:, -1]
-            _, f = warmstart.reprojection_frc(d_rp.numpy(), cfg["sino"], iomit, band)
-            fallback = int((R > R_plain).sum())
-            learned = dict(start_iterations=args.start, continuation_iterations=k_star,
-                           slices_per_s=round(value, 3), speedup_vs_plain=round(value / plain_value, 3),
-                           resid_mean=float(R.mean()), resid_max_over_target=round(float((R / R_plain).max()), 5),
-                           frc=round(f, 5), frc_plain=round(frc_plain, 5), frc_band=list(band),
-                           fallback_slices=fallback, train_steps=args.train_steps,
-                           train_loss_last=round(float(np.mean(losses[-50:])), 4), train_seconds=round(train_s, 1),
-                           unet_batch=args.unet_batch, no_network=nonet)
-            log("learned:", json.dumps(learned))
-            # the U-Net forward alone (the largest part of a learned step)
-            dt_u = timed(lambda: solver.net.warm_start_dev(S, solver.d_start, solver.d_init), "unet forward")
-            gf = unet_gflop_per_slice()
-            tf = gf * S * args.steps / dt_u / 1e3
-            unet_roofline = dict(bound="mfma", achieved=round(tf, 1), peak=BF16_DENSE_PEAK_TFLOPS, unit="TFLOP/s",
-                                 frac=round(tf / BF16_DENSE_PEAK_TFLOPS, 4),
-                                 kernel="U-Net forward, all layers (BF16 MFMA convolutions), batches of %d" % args.unet_batch,
-                                 ms_per_256_slices=round(dt_u / args.steps * 1e3 * 256 / S, 2),
-                                 gflop_per_slice=round(gf, 2))
-            log("unet forward:", json.dumps(unet_roofline))
-        solver.close()
+        learned, unet_roofline, lv = learned_leg(args, plan, cfg, d_sino, (pre, icon_it, post), R_plain, m_plain,
+                                                 frc_plain, plain_value, timed)
+        if lv is not None:
+            value, ms_step = lv
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline: %d slices x %d iterations" % (args.cpu_slices, args.cpu_iters))
-        cpu = cpu_baseline(cfg, args.cpu_slices, args.cpu_iters, iters, min(args.cpu_threads, os.cpu_count() or 1))
-        log("cpu baseline done")
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        log("cpu baseline: %d slices x %d iterations on %d threads" % (threads, iters, threads))
+        cpu = cpu_baseline(cfg, (pre, icon_it, post), threads)
+        log("cpu baseline:", json.dumps(cpu))
     workload = ("learned-start ICON vs plain ICON: %d slices of 512x512, 41 tilts +-60 deg step 3, SNR 0.05; "
-                "target = per-slice data residual of plain ICON (%d iterations %d,%d,%d, m >= 0) with reprojection "
-                "FRC within %.3f; sinograms resident in HBM, solves replayed as HIP graphs"
-                % (S, iters, pre, icon_it, post, args.frc_tol))
+                "target = per-slice data residual of plain ICON (%d iterations %d,%d,%d, m >= 0), the continuation "
+                "length calibrated on a held-out training-distribution phantom where every slice also keeps its FRC "
+                "against the phantom within %.3f of plain ICON's; U-Net trained on %d (%d, %d)-iteration pairs; "
+                "per-slice fallback to plain ICON timed in the step; sinograms resident in HBM, solves replayed as "
+                "HIP graphs" % (S, iters, pre, icon_it, post, args.frc_tol, args.train_pairs, args.start, iters))
     out = {
         "metric": METRIC,
         "value": round(value, 3),

@@ -7,6 +7,7 @@ computes on the CPU, and loading fails loudly when the library is missing.
 from .icon import (  # noqa: F401
     IconError,
     Plan,
+    Batch,
     DeviceArray,
     SolveOptions,
     lib,

@@ -322,16 +322,18 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   return ICON_OK;
 }
 
-int prepare_col(int n0, int C, int K) {
+int prepare_col(int n0, int n1, int C, int K) {
   // sets the LDS attribute outside any capture
-  if (!col_config_ok(n0, C, K))
-    return fail(ICON_ERR_ARG, "column tile %d unsupported for grid %d with %d window taps", C, n0, K);
+  if (!col_tile_ok(n0, n1, C, K))
+    return fail(ICON_ERR_ARG, "column tile %d unsupported for grid %d x %d with %d window taps", C, n0, n1, K);
   hipError_t e = set_col_attrs(n0, C, K);
   if (e != hipSuccess) return fail(ICON_ERR_DEVICE, "hipFuncSetAttribute: %s", hipGetErrorString(e));
   return ICON_OK;
 }
 
 }  // namespace
+
+void icon::set_last_error(const char* msg) { g_err = msg; }
 
 // ====================================================================== ABI
 extern "C" {
@@ -383,6 +385,13 @@ int icon_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 
 int icon_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return ICON_OK;
+}
+
+int icon_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!bytes) return ICON_OK;
+  if (!dst || !src) return fail(ICON_ERR_ARG, "icon_memcpy_d2d_async: null pointer");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return ICON_OK;
 }
 
@@ -498,7 +507,7 @@ int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, cons
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming);
     if (e != hipSuccess) return bail(fail(ICON_ERR_DEVICE, "stream/event create: %s", hipGetErrorString(e)));
   }
-  if ((rc = prepare_col(g.n0, p->bins.C, g.K))) return bail(rc);
+  if ((rc = prepare_col(g.n0, g.n1, p->bins.C, g.K))) return bail(rc);
   *out = p;
   return ICON_OK;
 }
@@ -536,7 +545,7 @@ int icon_plan_get_info(const icon_plan* p, icon_plan_info* info) {
 
 int icon_plan_set_col_block(icon_plan* p, int C) {
   if (!p) return fail(ICON_ERR_ARG, "icon_plan_set_col_block: null plan");
-  if (!col_config_ok(p->g.n0, C, p->g.K) || p->g.n1 % C != 0)
+  if (!col_tile_ok(p->g.n0, p->g.n1, C, p->g.K))
     return fail(ICON_ERR_ARG, "column tile %d unsupported for grid %d x %d", C, p->g.n0, p->g.n1);
   DevGuard dg(p->device);
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -545,7 +554,7 @@ int icon_plan_set_col_block(icon_plan* p, int C) {
   build_bins(p->g, C, p->bins);
   int rc = upload_bins(p);
   if (rc) return rc;
-  return prepare_col(p->g.n0, C, p->g.K);
+  return prepare_col(p->g.n0, p->g.n1, C, p->g.K);
 }
 
 int icon_nufft_forward_real(icon_plan* p, int S, const void* d_img, void* d_f, void* stream) {
@@ -798,7 +807,7 @@ int icon_plan_autotune(icon_plan* p, int S, int* col_block, double* ms_per_width
   for (int i = 0; i < 2; i++) {
     const int C = widths[i];
     if (ms_per_width) ms_per_width[i] = 0.0;
-    if (!col_config_ok(p->g.n0, C, p->g.K) || p->g.n1 % C != 0) continue;
+    if (!col_tile_ok(p->g.n0, p->g.n1, C, p->g.K)) continue;
     int rc = icon_plan_set_col_block(p, C);
     double tf = 0.0, ta = 0.0;
     if (!rc) rc = icon_time_kernel(p, ICON_K_COL_FWD, S, 3, &tf);

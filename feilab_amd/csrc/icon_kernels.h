@@ -103,6 +103,18 @@ struct ReprojArgs {
   float* out;            // [S][N0]
 };
 
+// Per-launch fields of a packed launch over several series (the argument
+// blocks of the series are read from a device table).
+struct RowFwdLaunch {
+  int cplx, update, data_type;
+  float thr;
+};
+
+struct CombineLaunch {
+  int mode, resid_index, alpha_index;
+  int with_resid, with_frep;   // 0: the series' resid / frep pointers are ignored
+};
+
 // While set (non-null) on the calling thread, the next launches of the
 // iteration kernels (row_fwd, col, combine, row_adj, sino) go through
 // hipExtLaunchKernel with these events, which then hold the kernel's own start
@@ -123,7 +135,26 @@ hipError_t launch_zero(float* p, size_t n, hipStream_t st);
 hipError_t launch_node_to_ent(const float2* f, float2* fw_ent, const int4* ent, int S, int M, int E, hipStream_t st);
 hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
 hipError_t launch_copy4(float4* dst, const float4* src, size_t n, hipStream_t st);
+// Packed launches: `n` blocks, block b running block items[b].y of the series
+// descs[items[b].x] (both tables in device memory).  Rows per block of the
+// row kernels (the unit of their items) by row length:
+int row_fwd_rows_per_block(int NR);
+int row_adj_rows_per_block(int NR);
+int sino_rows_per_block(int NR);
+hipError_t launch_row_fwd_multi(int NR, const RowFwdArgs* descs, const int2* items, int n, const RowFwdLaunch& L,
+                                hipStream_t st);
+hipError_t launch_row_adj_multi(int NR, const RowAdjArgs* descs, const int2* items, int n, int mode, bool fwd,
+                                hipStream_t st);
+hipError_t launch_col_multi(int NC, int C, int K, bool adjoint, const ColArgs* descs, const int2* items, int n,
+                            hipStream_t st);
+hipError_t launch_combine_multi(const CombineArgs* descs, const int2* items, int n, const CombineLaunch& L,
+                                hipStream_t st);
+hipError_t launch_sino_multi(int NR, const SinoArgs* descs, const int2* items, int n, hipStream_t st);
+hipError_t launch_reproj_multi(int NR, const ReprojArgs* descs, const int2* items, int n, hipStream_t st);
 bool col_config_ok(int NC, int C, int K);
+// A column tile of C grid columns usable on an n0 x n1 grid with K taps.
+bool col_tile_ok(int NC, int n1, int C, int K);
 hipError_t set_col_attrs(int NC, int C, int K);
+hipError_t set_col_multi_attrs(int NC, int C, int K);
 
 }  // namespace icon

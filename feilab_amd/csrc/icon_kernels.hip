@@ -105,12 +105,12 @@ __device__ __forceinline__ void row_fwd_inputs(float2 (&v)[16], const RowFwdArgs
 }
 
 template <int NR, int RPB>
-__global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
+__device__ __forceinline__ void row_fwd_body(const RowFwdArgs& a, int bid) {
   constexpr int T = NR / 16;
   __shared__ float2 lds[RPB * kRowLds<NR>];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int row = blockIdx.x * RPB + team;
+  const int row = bid * RPB + team;
   const bool valid = row < a.rows;
   const int s = valid ? row / a.N0 : 0;
   const int X = valid ? row % a.N0 : 0;
@@ -132,6 +132,23 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
 #pragma unroll
     for (int i = 0; i < 16; i++) o[out_index<NR>(t, i)] = v[i];
   }
+}
+
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd(RowFwdArgs a) {
+  row_fwd_body<NR, RPB>(a, blockIdx.x);
+}
+
+// Packed launch over several series: block b runs block items[b].y of the
+// series whose argument block is descs[items[b].x]; the per-launch fields
+// (image type, update, threshold) come by value.
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_row_fwd_multi(const RowFwdArgs* __restrict__ descs,
+                                                                  const int2* __restrict__ items, RowFwdLaunch L) {
+  const int2 it = items[blockIdx.x];
+  RowFwdArgs a = descs[it.x];
+  a.cplx = L.cplx; a.update = L.update; a.data_type = L.data_type; a.thr = L.thr;
+  row_fwd_body<NR, RPB>(a, it.y);
 }
 
 
@@ -190,13 +207,13 @@ __device__ __forceinline__ void row_adj_outputs(const float2 (&v)[16], float2 (&
 }
 
 template <int NR, int RPB, bool FWD>
-__global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
+__device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
   constexpr int T = NR / 16;
   __shared__ float2 lds[RPB * kRowLds<NR>];
   __shared__ double red[RPB * T];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int row = blockIdx.x * RPB + team;
+  const int row = bid * RPB + team;
   const bool valid = row < a.rows;
   const int s = valid ? row / a.N0 : 0;
   const int X = valid ? row % a.N0 : 0;
@@ -235,6 +252,21 @@ __global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
       for (int i = 0; i < 16; i++) o[out_index<NR>(t, i)] = u[i];
     }
   }
+}
+
+template <int NR, int RPB, bool FWD>
+__global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
+  row_adj_body<NR, RPB, FWD>(a, blockIdx.x);
+}
+
+template <int NR, int RPB, bool FWD>
+__global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj_multi(const RowAdjArgs* __restrict__ descs,
+                                                                     const int2* __restrict__ items, int mode) {
+  const int2 it = items[blockIdx.x];
+  RowAdjArgs a = descs[it.x];
+  a.mode = mode;
+  if (!FWD) a.fwd_out = nullptr;
+  row_adj_body<NR, RPB, FWD>(a, it.y);
 }
 
 // --------------------------------------------------------------- column pass
@@ -290,14 +322,23 @@ __host__ __device__ constexpr int col_lds() {
 }
 
 template <int NC, int C, int K>
-__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
+__device__ __forceinline__ void col_fwd_body(const ColArgs& a, int bid) {
   constexpr int NT = C * (NC / 16);
   constexpr int LS = col_stride<NC, C>();
   extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
-  // tile-major order: blocks in flight together work on the same column
-  // tile of different slices and share its bin records in L2
-  const int blk = blockIdx.x / a.S;
-  const int s = blockIdx.x % a.S;
+  // Block order.  A tile's grid rows are 8 C bytes of each 128-byte line, so
+  // the tiles 2p and 2p + 1 of a slice read the two halves of the same lines:
+  // they run as blocks b and b + 8, which the dispatcher places on the same
+  // XCD (workgroups go round-robin over the 8 XCDs) one after the other, so
+  // the second half comes from that XCD's L2 instead of a second HBM read.
+  // Each group of 16 blocks takes 8 consecutive (slice, tile pair) items;
+  // items are tile-pair-major, so blocks in flight together also share the
+  // pair's bin records in L2.  S * n1 / C is a multiple of 16 (n1 / C is a
+  // power of two >= 16).
+  const int item = (bid >> 4) * 8 + (bid & 7);
+  const int pair = item / a.S;
+  const int s = item % a.S;
+  const int blk = 2 * pair + ((bid >> 3) & 1);
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int h0 = a.N0 >> 1;
@@ -405,7 +446,20 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
 }
 
 template <int NC, int C, int K>
-__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
+__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd(ColArgs a) {
+  col_fwd_body<NC, C, K>(a, blockIdx.x);
+}
+
+template <int NC, int C, int K>
+__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_fwd_multi(const ColArgs* __restrict__ descs,
+                                                                   const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const ColArgs a = descs[it.x];
+  col_fwd_body<NC, C, K>(a, it.y);
+}
+
+template <int NC, int C, int K>
+__device__ __forceinline__ void col_adj_body(const ColArgs& a, int bid) {
   constexpr int NT = C * (NC / 16);
   constexpr int RL = NC / NT;                    // adjacent grid rows per thread
   static_assert(RL >= 1 && RL <= 2 && NT * RL == NC, "one or two grid rows per thread");
@@ -418,8 +472,8 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
   // tile-major order: blocks in flight together work on the same column
   // tile of different slices and share its bin records in L2
-  const int blk = blockIdx.x / a.S;
-  const int s = blockIdx.x % a.S;
+  const int blk = bid / a.S;
+  const int s = bid % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   // a tile no node reaches: its spread grid and so its transform are zero
@@ -545,6 +599,19 @@ __global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
   }
 }
 
+template <int NC, int C, int K>
+__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj(ColArgs a) {
+  col_adj_body<NC, C, K>(a, blockIdx.x);
+}
+
+template <int NC, int C, int K>
+__global__ __launch_bounds__(C * (NC / 16), 4) void k_col_adj_multi(const ColArgs* __restrict__ descs,
+                                                                   const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const ColArgs a = descs[it.x];
+  col_adj_body<NC, C, K>(a, it.y);
+}
+
 // ------------------------------------------------------------ node combine
 // One block (1024 threads) per slice.  FP64 block reductions in a fixed order.
 
@@ -563,9 +630,8 @@ __device__ __forceinline__ double block_sum_1024(double v, double* red) {
   return r;
 }
 
-__global__ __launch_bounds__(1024) void k_combine(CombineArgs a) {
+__device__ __forceinline__ void combine_body(const CombineArgs& a, int s) {
   __shared__ double red[1024];
-  const int s = blockIdx.x;
   const int tid = threadIdx.x;
   double acc = 0.0;
   const float2* P = a.part + (size_t)s * a.M * kMaxSlotK;
@@ -633,17 +699,33 @@ __global__ __launch_bounds__(1024) void k_combine(CombineArgs a) {
   }
 }
 
+__global__ __launch_bounds__(1024) void k_combine(CombineArgs a) { combine_body(a, blockIdx.x); }
+
+// Packed launch: the per-launch mode and trace indices by value; with_resid /
+// with_frep switch the series' residual trace and reprojection values on.
+__global__ __launch_bounds__(1024) void k_combine_multi(const CombineArgs* __restrict__ descs,
+                                                        const int2* __restrict__ items, CombineLaunch L) {
+  const int2 it = items[blockIdx.x];
+  CombineArgs a = descs[it.x];
+  a.mode = L.mode;
+  a.resid_index = L.resid_index;
+  a.alpha_index = L.alpha_index;
+  if (!L.with_resid) a.resid = nullptr;
+  if (!L.with_frep) a.frep = nullptr;
+  combine_body(a, it.y);
+}
+
 // ------------------------------------------------------- projection rows (1D)
 // One team of NR/16 threads per (slice, tilt) row; RPB rows per block.
 
 
 template <int NR, int RPB>
-__global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
+__device__ __forceinline__ void sino_body(const SinoArgs& a, int bid) {
   constexpr int T = NR / 16;
   __shared__ float2 lds[RPB * kRowLds<NR>];
   const int team = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int row = blockIdx.x * RPB + team;
+  const int row = bid * RPB + team;
   const bool valid = row < a.rows;
   const int h = a.N0 >> 1;
   // node row i of slice s reads input row i, or (cross validation) the rows
@@ -690,15 +772,27 @@ __global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
   }
 }
 
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_sino(SinoArgs a) {
+  sino_body<NR, RPB>(a, blockIdx.x);
+}
+
+template <int NR, int RPB>
+__global__ __launch_bounds__(RPB * (NR / 16)) void k_sino_multi(const SinoArgs* __restrict__ descs,
+                                                                const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const SinoArgs a = descs[it.x];
+  sino_body<NR, RPB>(a, it.y);
+}
+
 // --------------------------------------------------------- reprojection (1D)
 // One team of NR/16 threads per slice.  The 1D spread is a gather per grid
 // cell over the nodes that touch it, in ascending node order.
 
 
 template <int NR>
-__global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
+__device__ __forceinline__ void reproj_body(const ReprojArgs& a, int s) {
   __shared__ float2 lds[kRowLds<NR>];
-  const int s = blockIdx.x;
   const int t = threadIdx.x;
   const float2* f = a.frep + (size_t)s * a.N0;
   float2 v[16];
@@ -737,6 +831,19 @@ __global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
     const int X = (y + h) & (NR - 1);
     if (X < a.N0) a.out[(size_t)s * a.N0 + X] = v[i].x / a.ck1[X];
   }
+}
+
+template <int NR>
+__global__ __launch_bounds__(NR / 16) void k_reproj(ReprojArgs a) {
+  reproj_body<NR>(a, blockIdx.x);
+}
+
+template <int NR>
+__global__ __launch_bounds__(NR / 16) void k_reproj_multi(const ReprojArgs* __restrict__ descs,
+                                                         const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const ReprojArgs a = descs[it.x];
+  reproj_body<NR>(a, it.y);
 }
 
 // -------------------------------------------------------------- plain FFTs
@@ -939,6 +1046,11 @@ hipError_t launch_fft_rows(int NR, float2* data, int rows, int sign, const float
   X(1024, 16, 14) X(2048, 8, 14) X(128, 8, 6) X(256, 8, 6) X(512, 8, 6) X(1024, 8, 6) X(2048, 8, 6)
 #define FFT_COL_CASES(X) X(128, 8) X(128, 16) X(256, 8) X(256, 16) X(512, 8) X(512, 16) X(1024, 8) X(1024, 16) X(2048, 8)
 
+bool col_tile_ok(int NC, int n1, int C, int K) {
+  // the interpolation's block order pairs tiles in groups of 16 blocks
+  return col_config_ok(NC, C, K) && n1 % C == 0 && (n1 / C) % 16 == 0;
+}
+
 bool col_config_ok(int NC, int C, int K) {
 #define CASE(N, CC, KK) if (NC == N && C == CC && K == KK) return true;
   COL_CASES(CASE)
@@ -984,6 +1096,7 @@ hipError_t set_col_attrs(int NC, int C, int K) {
 hipError_t launch_col(int NC, int C, bool adjoint, const ColArgs& a, hipStream_t st) {
   const ColArgs& h = a;
   const int nblk = h.n1 / C;
+  if (!col_tile_ok(NC, h.n1, C, h.K)) return hipErrorInvalidValue;
   hipError_t e = hipErrorInvalidValue;
 #define CASE(N, CC, KK)                                                                                  \
   if (NC == N && C == CC && h.K == KK) {                                                                 \
@@ -1016,6 +1129,146 @@ hipError_t launch_fft_cols(int NC, int C, float2* data, int n1, int count, int s
   FFT_COL_CASES(CASE)
 #undef CASE
   return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------- packed launches
+
+int row_fwd_rows_per_block(int NR) {
+  switch (NR) {
+#define CASE(N) case N: return rows_per_block<N>();
+    ROW_CASES(CASE)
+#undef CASE
+    default: return 0;
+  }
+}
+
+int row_adj_rows_per_block(int NR) {
+  switch (NR) {
+#define CASE(N) case N: return row_adj_rows_per_block<N>();
+    ROW_CASES(CASE)
+#undef CASE
+    default: return 0;
+  }
+}
+
+int sino_rows_per_block(int NR) { return row_fwd_rows_per_block(NR); }
+
+hipError_t launch_row_fwd_multi(int NR, const RowFwdArgs* descs, const int2* items, int n, const RowFwdLaunch& L,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (NR) {
+#define CASE(N)                                                                                   \
+  case N: {                                                                                       \
+    constexpr int RPB = rows_per_block<N>();                                                      \
+    LAUNCH_TIMED((k_row_fwd_multi<N, RPB>), dim3(n), dim3(RPB * (N / 16)), 0, st, descs, items, L); \
+    break;                                                                                        \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_row_adj_multi(int NR, const RowAdjArgs* descs, const int2* items, int n, int mode, bool fwd,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (NR) {
+#define CASE(N)                                                                                         \
+  case N: {                                                                                             \
+    constexpr int RPB = row_adj_rows_per_block<N>();                                                    \
+    if (fwd)                                                                                            \
+      LAUNCH_TIMED((k_row_adj_multi<N, RPB, true>), dim3(n), dim3(RPB * (N / 16)), 0, st, descs, items, mode); \
+    else                                                                                                \
+      LAUNCH_TIMED((k_row_adj_multi<N, RPB, false>), dim3(n), dim3(RPB * (N / 16)), 0, st, descs, items, mode); \
+    break;                                                                                              \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sino_multi(int NR, const SinoArgs* descs, const int2* items, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (NR) {
+#define CASE(N)                                                                                \
+  case N: {                                                                                    \
+    constexpr int RPB = rows_per_block<N>();                                                   \
+    LAUNCH_TIMED((k_sino_multi<N, RPB>), dim3(n), dim3(RPB * (N / 16)), 0, st, descs, items);    \
+    break;                                                                                     \
+  }
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_reproj_multi(int NR, const ReprojArgs* descs, const int2* items, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (NR) {
+#define CASE(N)                                                                          \
+  case N:                                                                                \
+    hipLaunchKernelGGL((k_reproj_multi<N>), dim3(n), dim3(N / 16), 0, st, descs, items); \
+    break;
+    ROW_CASES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int NC, int C, int K>
+static hipError_t col_multi_attr() {
+  static bool done = false;
+  if (!done) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_col_fwd_multi<NC, C, K>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, col_lds<NC, C>() * (int)sizeof(float2));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_col_adj_multi<NC, C, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            col_lds<NC, C>() * (int)sizeof(float2));
+    if (e != hipSuccess) return e;
+    done = true;
+  }
+  return hipSuccess;
+}
+
+hipError_t set_col_multi_attrs(int NC, int C, int K) {
+#define CASE(N, CC, KK) \
+  if (NC == N && C == CC && K == KK) return col_multi_attr<N, CC, KK>();
+  COL_CASES(CASE)
+#undef CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_col_multi(int NC, int C, int K, bool adjoint, const ColArgs* descs, const int2* items, int n,
+                            hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipError_t e = hipErrorInvalidValue;
+#define CASE(N, CC, KK)                                                                                   \
+  if (NC == N && C == CC && K == KK) {                                                                    \
+    e = col_multi_attr<N, CC, KK>();                                                                      \
+    if (e != hipSuccess) return e;                                                                        \
+    if (adjoint)                                                                                          \
+      LAUNCH_TIMED((k_col_adj_multi<N, CC, KK>), dim3(n), dim3(CC * (N / 16)),                            \
+                   (col_lds<N, CC>()) * sizeof(float2), st, descs, items);                                \
+    else                                                                                                  \
+      LAUNCH_TIMED((k_col_fwd_multi<N, CC, KK>), dim3(n), dim3(CC * (N / 16)),                            \
+                   (col_lds<N, CC>()) * sizeof(float2), st, descs, items);                                \
+    return hipGetLastError();                                                                             \
+  }
+  COL_CASES(CASE)
+#undef CASE
+  return e;
+}
+
+hipError_t launch_combine_multi(const CombineArgs* descs, const int2* items, int n, const CombineLaunch& L,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  LAUNCH_TIMED(k_combine_multi, dim3(n), dim3(1024), 0, st, descs, items, L);
+  return hipGetLastError();
 }
 
 hipError_t launch_combine(const CombineArgs& a, hipStream_t st) {

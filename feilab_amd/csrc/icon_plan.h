@@ -61,6 +61,8 @@ bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, 
                     int window = ICON_WINDOW_GAUSS);
 void build_bins(const Geometry& g, int C, Bins& b);
 void grid_size(int N, float* sigma, int* n);
+// Sets the message icon_last_error() returns on the calling thread.
+void set_last_error(const char* msg);
 
 struct GraphKey {
   int S = -1;

@@ -44,3 +44,38 @@ def frc(image1, image2):
     out = np.where(den != 0, U / np.where(den != 0, den, 1), 0.0)
     out[0] = 1.0
     return out
+
+
+def frc_slices(images1, images2):
+    """FRC curve of every pair of slices of two stacks [S][ny][nx]: [S][nx/2]
+    (each slice on its own, not pooled as frc() pools a stack)."""
+    a = np.asarray(images1, np.float32)
+    b = np.asarray(images2, np.float32)
+    S, ny, nx = a.shape
+    if nx < ny:
+        raise ValueError("frc: width must be at least the height (calFRC_nonsqure)")
+    r = _rings(nx, ny)
+    keep = r < nx // 2
+    idx = r[keep]
+    nr = nx // 2
+    out = np.zeros((S, nr))
+    try:
+        from scipy import fft as sfft
+        fft2 = lambda x: sfft.fft2(x, workers=-1)
+    except ImportError:
+        fft2 = np.fft.fft2
+    for s0 in range(0, S, 16):
+        A = fft2(a[s0:s0 + 16].astype(np.float64))
+        B = fft2(b[s0:s0 + 16].astype(np.float64))
+        k = A.shape[0]
+        up = (A * np.conj(B)).real[:, keep]
+        l1 = (np.abs(A) ** 2)[:, keep]
+        l2 = (np.abs(B) ** 2)[:, keep]
+        flat = (np.arange(k)[:, None] * nr + idx[None, :]).ravel()
+        U = np.bincount(flat, up.ravel(), k * nr).reshape(k, nr)
+        L1 = np.bincount(flat, l1.ravel(), k * nr).reshape(k, nr)
+        L2 = np.bincount(flat, l2.ravel(), k * nr).reshape(k, nr)
+        den = np.sqrt(L1) * np.sqrt(L2)
+        out[s0:s0 + k] = np.where(den != 0, U / np.where(den != 0, den, 1), 0.0)
+    out[:, 0] = 1.0
+    return out

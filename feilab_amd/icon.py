@@ -80,6 +80,7 @@ def lib():
         "icon_memcpy_h2d": [vp, vp, cs],
         "icon_memcpy_d2h": [vp, vp, cs],
         "icon_memset_d": [vp, ci, cs],
+        "icon_memcpy_d2d_async": [vp, vp, cs, vp],
         "icon_device_synchronize": [],
         "icon_plan_create": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
         "icon_plan_create_cv": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
@@ -114,6 +115,11 @@ def lib():
         "icon_mrc_write_slice": [ctypes.c_char_p, ctypes.c_char_p, vp, ci, ci],
         "icon_time_kernel": [vp, ci, ci, ci, ctypes.POINTER(ctypes.c_double)],
         "icon_copy_probe": [vp, vp, cs, ci, ctypes.POINTER(ctypes.c_double)],
+        "icon_batch_create": [ctypes.POINTER(vp), ci, vp, vp],
+        "icon_batch_destroy": [vp],
+        "icon_batch_reconstruct": [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolveOptions), vp],
+        "icon_batch_launches_per_iteration": [vp, ctypes.POINTER(ci)],
+        "icon_batch_autotune": [vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -172,6 +178,12 @@ class DeviceArray:
                    "icon_memcpy_d2h")
         return out
 
+    def addr(self, offset_bytes=0):
+        """Device address offset_bytes into the buffer (for copies of parts)."""
+        if not 0 <= offset_bytes <= self.nbytes:
+            raise ValueError("offset %d outside a buffer of %d bytes" % (offset_bytes, self.nbytes))
+        return self.ptr.value + offset_bytes
+
     def free(self):
         if self.ptr and self.ptr.value and _lib is not None:
             _lib.icon_free(self.ptr)
@@ -182,6 +194,11 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def copy_d2d(dst, src, nbytes, stream=None):
+    """Asynchronous device-to-device copy between device addresses (ints or DeviceArrays)."""
+    _check(lib().icon_memcpy_d2d_async(_ptr(dst), _ptr(src), int(nbytes), _ptr(stream)), "icon_memcpy_d2d_async")
 
 
 def _ptr(x):
@@ -327,6 +344,77 @@ class Plan:
             out.append(rs[0] if single else rs)
             out.append((al[0] if single else al)[..., :iters])
         return out[0] if len(out) == 1 else tuple(out)
+
+
+class Batch:
+    """Several series (one Plan each, same device) solved in the same packed
+    launches: icon_batch_create / icon_batch_reconstruct.  slices[i] slices of
+    series i are solved by every call."""
+
+    def __init__(self, plans, slices):
+        self.plans = list(plans)
+        self.slices = [int(v) for v in slices]
+        n = len(self.plans)
+        arr = (ctypes.c_void_p * n)(*[p.h.value for p in self.plans])
+        sl = (ctypes.c_int * n)(*self.slices)
+        self.h = ctypes.c_void_p()
+        _check(lib().icon_batch_create(ctypes.byref(self.h), n, arr, sl), "icon_batch_create")
+
+    def close(self):
+        if self.h and self.h.value and _lib is not None:
+            _lib.icon_batch_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arr(xs, n):
+        if xs is None:
+            return None
+        return (ctypes.c_void_p * n)(*[None if x is None else (x.ptr.value if isinstance(x, DeviceArray) else int(x))
+                                       for x in xs])
+
+    def reconstruct_dev(self, d_sinos, d_ms, opts, d_m_inits=None, d_reprojs=None, d_resids=None, d_alphas=None,
+                        stream=None):
+        n = len(self.plans)
+        A = lambda xs: self._arr(xs, n)
+        _check(lib().icon_batch_reconstruct(self.h, A(d_sinos), A(d_ms), A(d_m_inits), A(d_reprojs), A(d_resids),
+                                            A(d_alphas), ctypes.byref(opts), _ptr(stream)), "icon_batch_reconstruct")
+
+    def launches_per_iteration(self):
+        v = ctypes.c_int(0)
+        _check(lib().icon_batch_launches_per_iteration(self.h, ctypes.byref(v)), "icon_batch_launches_per_iteration")
+        return v.value
+
+    def autotune(self):
+        """Column tile per size class; returns the tile chosen for each series."""
+        out = (ctypes.c_int * len(self.plans))()
+        _check(lib().icon_batch_autotune(self.h, out), "icon_batch_autotune")
+        for p, c in zip(self.plans, out):
+            p.info.col_block = c
+        return list(out)
+
+    def solve(self, sinos, pre, icon, post, data_type=2, threshold=0.0, trace=False, reproject=False):
+        """Host arrays in and out: sinos[i] [slices[i]][nz_i][N0_i] -> images
+        (and residual traces [S][iters+1], reprojections [S][N0]) per series."""
+        iters = pre + icon + post
+        ds, dm, dr, dp = [], [], [], []
+        for p, S, x in zip(self.plans, self.slices, sinos):
+            ds.append(DeviceArray.from_host(np.asarray(x, np.float32).reshape(S, p.nz, p.N0)))
+            dm.append(DeviceArray((S, p.N0, p.N1), np.float32))
+            dr.append(DeviceArray((S, iters + 1), np.float64) if trace else None)
+            dp.append(DeviceArray((S, p.N0), np.float32) if reproject else None)
+        self.reconstruct_dev(ds, dm, SolveOptions(pre, icon, post, data_type, threshold, 1, 0),
+                             d_resids=dr if trace else None, d_reprojs=dp if reproject else None)
+        lib().icon_device_synchronize()
+        out = [d.numpy() for d in dm]
+        res = [d.numpy() for d in dr] if trace else None
+        rps = [d.numpy() for d in dp] if reproject else None
+        return out, res, rps
 
 
 def fft_rows(x, sign=-1):

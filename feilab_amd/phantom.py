@@ -128,3 +128,36 @@ def render_slices_3d(b, N0, N1, slices):
         gz = np.exp(-(np.arange(z0, z1) - cz) ** 2 / (2 * s * s))
         out[keep, x0:x1, z0:z1] += wy[keep, None, None] * (gx[:, None] * gz[None, :])[None]
     return out.astype(np.float32)
+
+
+def config2_sino(nslices_seed):
+    """Sinograms of config2(nslices, seed) for (nslices, seed): the unit of work
+    of a process pool generating training series."""
+    n, seed = nslices_seed
+    return config2(nslices=n, seed=seed)["sino"]
+
+
+def config3_specs(seed=1003, nseries=48):
+    """The uneven batch: nseries tilt series with width U{128..1024} (the
+    thickness equals the width), slice count U{32..300}, maximum tilt
+    U(45, 70) deg, tilt step U{1..4} deg and SNR U(0.02, 0.2), seeded."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nseries):
+        N = int(rng.integers(128, 1025))
+        S = int(rng.integers(32, 301))
+        mt = float(rng.uniform(45.0, 70.0))
+        st = int(rng.integers(1, 5))
+        snr = float(rng.uniform(0.02, 0.2))
+        out.append(dict(N=N, S=S, max_tilt=mt, step=st, snr=snr, tilt=tilt_series(mt, st), seed=seed * 100 + i))
+    return out
+
+
+def config3_sino(spec, slices=None):
+    """Sinograms [S][nz][N] of one config-3 series: a blob phantom of N x N x S
+    (blob count scaled with the area, 400 at 512^2) projected analytically at
+    the series' tilts, noise at its SNR.  slices: the first `slices` only."""
+    N, S = spec["N"], spec["S"] if slices is None else min(slices, spec["S"])
+    rng = np.random.default_rng(spec["seed"])
+    b = blobs_3d(N, S, N, max(20, int(400 * (N / 512.0) ** 2)), rng)
+    return add_noise(project_3d(b, N, np.arange(S), spec["tilt"]), spec["snr"], rng)

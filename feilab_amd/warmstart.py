@@ -13,12 +13,16 @@ ICON's.  Everything runs on the device through the C ABI; plain ICON
 Training pairs are (m_start, m_final) from the device solve itself: the
 continuation of an iterate under the rest of the schedule is the uninterrupted
 solve, because the iteration state is m alone.
+
+The fallback is per slice: LearnedSolver.solve checks every slice's final
+data residual against its target and re-solves the slices that miss it with
+plain ICON from m = 0, replacing their image, reprojection and residual.
 """
 import math
 
 import numpy as np
 
-from .icon import DeviceArray, SolveOptions, lib
+from .icon import DeviceArray, SolveOptions, copy_d2d, lib
 from .unet import UNet
 
 
@@ -41,10 +45,15 @@ def iterate_pair(plan, d_sino, S, start, schedule, data_type=2, threshold=0.0):
 def normalise(m_start, m_final):
     """Network input m_start / s and target (m_final - m_start) / s, s the RMS
     of each m_start slice (1 for a zero slice): the scaling of the warm start."""
-    s = np.sqrt((m_start.astype(np.float64) ** 2).mean(axis=(1, 2)))
+    m_start = np.asarray(m_start, np.float32)
+    m_final = np.asarray(m_final, np.float32)
+    n = m_start.shape[1] * m_start.shape[2]
+    s = np.sqrt(np.square(m_start).sum(axis=(1, 2), dtype=np.float64) / n)
     s[s == 0] = 1.0
-    x = (m_start / s[:, None, None]).astype(np.float32)
-    t = ((m_final.astype(np.float64) - m_start) / s[:, None, None]).astype(np.float32)
+    inv = (1.0 / s).astype(np.float32)[:, None, None]
+    x = m_start * inv
+    t = np.subtract(m_final, m_start)
+    t *= inv
     return x, t
 
 
@@ -91,6 +100,46 @@ class Trainer:
         return self.losses
 
 
+def make_pairs(plan, sino_chunks, total, start, schedule, data_type=2, threshold=0.0, log=None):
+    """Normalised training pairs (x, t), float32 [total][N0][N1], from tilt
+    series chunks ([s][nz][N0] arrays, s <= plan.max_slices, `total` slices
+    in all): each chunk is solved on the device to its start and final
+    iterates, normalised and stored, so only one chunk's iterates are held
+    beyond the pairs themselves."""
+    x = np.empty((total, plan.N0, plan.N1), np.float32)
+    t = np.empty((total, plan.N0, plan.N1), np.float32)
+    k, nchunk = 0, 0
+    for sino in sino_chunks:
+        sino = np.ascontiguousarray(sino, np.float32)
+        S = sino.shape[0]
+        if k + S > total:
+            raise ValueError("the chunks hold more than %d slices" % total)
+        d_s = DeviceArray.from_host(sino)
+        m0, m1 = iterate_pair(plan, d_s, S, start, schedule, data_type, threshold)
+        d_s.free()
+        x[k:k + S], t[k:k + S] = normalise(m0, m1)
+        k += S
+        nchunk += 1
+        if log and nchunk % 16 == 0:
+            log("pairs: %d of %d" % (k, total))
+    if k != total:
+        raise ValueError("the chunks held %d slices, not %d" % (k, total))
+    return x, t
+
+
+def train_on_pairs(x, t, steps=1500, patch=128, batch=16, lr=1e-3, seed=0, device=0, log=None):
+    """Train a U-Net on normalised pairs (make_pairs); returns (params, losses)."""
+    net = UNet(patch, patch, max_batch=batch, train=True, device=device, seed=seed + 1)
+    tr = Trainer(net, x, t, patch, batch, seed)
+    tr.train(steps, lr)
+    if log:
+        log("trained %d steps on %d pairs: loss %.4g -> %.4g"
+            % (steps, x.shape[0], np.mean(tr.losses[:20]), np.mean(tr.losses[-50:])))
+    p = net.params()
+    net.close()
+    return p, tr.losses
+
+
 def train_warm_start(plan, sino, start, schedule, steps=1500, patch=128, batch=16, lr=1e-3, seed=0,
                      data_type=2, threshold=0.0, device=0, log=None):
     """Train a U-Net on (start-iterate, final-iterate) pairs of the sinograms
@@ -111,15 +160,23 @@ def train_warm_start(plan, sino, start, schedule, steps=1500, patch=128, batch=1
 
 
 class LearnedSolver:
-    """Learned-start reconstruction of up to max_slices slices on a plan."""
+    """Learned-start reconstruction of up to max_slices slices on a plan, with
+    plain ICON (schedule = (pre, icon, post)) as the per-slice fallback."""
 
-    def __init__(self, plan, params, max_slices, start=4, data_type=2, threshold=0.0, device=0, batch=128):
+    def __init__(self, plan, params, max_slices, start=10, schedule=(20, 160, 20), data_type=2, threshold=0.0,
+                 device=0, batch=128):
         self.plan, self.start, self.data_type, self.threshold = plan, start, data_type, threshold
+        self.schedule = tuple(int(v) for v in schedule)
         self.net = UNet(plan.N0, plan.N1, max_batch=min(batch, max_slices), device=device)
         self.net.set_params(params)
         shape = (max_slices, plan.N0, plan.N1)
         self.d_start = DeviceArray(shape, np.float32)
         self.d_init = DeviceArray(shape, np.float32)
+        iters = sum(self.schedule)
+        self.d_fb_sino = DeviceArray((max_slices, plan.nz, plan.N0), np.float32)
+        self.d_fb_m = DeviceArray(shape, np.float32)
+        self.d_fb_rp = DeviceArray((max_slices, plan.N0), np.float32)
+        self.d_fb_res = DeviceArray((max_slices, iters + 1), np.float64)
 
     def run(self, S, d_sino, d_m, cont, d_resid=None, d_reproj=None, stream=None, use_net=True):
         """start plain iterations -> U-Net -> `cont` descent iterations into d_m;
@@ -135,8 +192,67 @@ class LearnedSolver:
                           d_m_init=self.d_init if use_net else self.d_start, d_resid=d_resid, d_reproj=d_reproj,
                           stream=stream)
 
+    def solve(self, S, d_sino, d_m, cont, target, d_resid, d_reproj=None):
+        """The learned solve with its fallback: run(), then every slice whose
+        final R exceeds target[s] is re-solved by plain ICON from m = 0 and
+        its image, reprojection and final residual (d_resid[s][cont]) are
+        replaced.  Synchronises; returns the indices of the re-solved slices."""
+        self.run(S, d_sino, d_m, cont, d_resid=d_resid, d_reproj=d_reproj)
+        R = d_resid.numpy().reshape(-1)[:S * (cont + 1)].reshape(S, cont + 1)[:, cont]
+        bad = np.nonzero(R > np.asarray(target, np.float64)[:S])[0]
+        if bad.size:
+            self._fallback(bad, d_sino, d_m, cont, d_resid, d_reproj)
+            lib().icon_device_synchronize()
+        return bad
+
+    def _fallback(self, bad, d_sino, d_m, cont, d_resid, d_reproj):
+        p = self.plan
+        pre, icon_it, post = self.schedule
+        iters = pre + icon_it + post
+        row, img, line = p.nz * p.N0 * 4, p.N0 * p.N1 * 4, p.N0 * 4
+        for i, s in enumerate(bad):
+            copy_d2d(self.d_fb_sino.addr(i * row), d_sino.addr(int(s) * row), row)
+        p.reconstruct_dev(len(bad), self.d_fb_sino, self.d_fb_m,
+                          SolveOptions(pre, icon_it, post, self.data_type, self.threshold, 0, 0),
+                          d_reproj=self.d_fb_rp if d_reproj is not None else None, d_resid=self.d_fb_res)
+        for i, s in enumerate(bad):
+            s = int(s)
+            copy_d2d(d_m.addr(s * img), self.d_fb_m.addr(i * img), img)
+            if d_reproj is not None:
+                copy_d2d(d_reproj.addr(s * line), self.d_fb_rp.addr(i * line), line)
+            copy_d2d(d_resid.addr((s * (cont + 1) + cont) * 8), self.d_fb_res.addr((i * (iters + 1) + iters) * 8), 8)
+
     def close(self):
         self.net.close()
+        for a in (self.d_start, self.d_init, self.d_fb_sino, self.d_fb_m, self.d_fb_rp, self.d_fb_res):
+            a.free()
+
+
+def calibrate_continuation(solver, S, d_sino, target, frc_target=None, sino=None, frc_tol=0.005, max_cont=40,
+                           band=None):
+    """Smallest continuation k for which every slice of a calibration batch
+    reaches R <= target[s] (and, given frc_target, the stack's reprojection
+    FRC over `band` lies within frc_tol of it); None if no k <= max_cont does."""
+    p = solver.plan
+    d_r = DeviceArray((S, max_cont + 1), np.float64)
+    d_rp = DeviceArray((S, p.N0), np.float32)
+    d_m = DeviceArray((S, p.N0, p.N1), np.float32)
+    try:
+        for k in range(1, max_cont + 1):
+            solver.run(S, d_sino, d_m, k, d_resid=d_r, d_reproj=d_rp)
+            lib().icon_device_synchronize()
+            R = d_r.numpy().reshape(-1)[:S * (k + 1)].reshape(S, k + 1)[:, k]
+            if not (R <= np.asarray(target)[:S]).all():
+                continue
+            if frc_target is not None:
+                _, f = reprojection_frc(d_rp.numpy(), sino, p.iomit, band)
+                if abs(f - frc_target) > frc_tol:
+                    continue
+            return k
+        return None
+    finally:
+        for a in (d_r, d_rp, d_m):
+            a.free()
 
 
 def reprojection_frc(reproj, sino, iomit, band):

@@ -96,6 +96,8 @@ int icon_free(void* dptr);
 int icon_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int icon_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int icon_memset_d(void* dst, int value, size_t bytes);
+/* Device-to-device copy, asynchronous on `stream` (NULL = default stream). */
+int icon_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
 int icon_device_synchronize(void);
 
 /* ---- plans ------------------------------------------------------------- */
@@ -169,6 +171,38 @@ int icon_reconstruct(icon_plan* plan, int S, const void* d_sino, void* d_m, cons
 /* Host-buffer convenience wrapper of icon_reconstruct (synchronous). */
 int icon_reconstruct_host(icon_plan* plan, int S, const float* sino, float* m, const float* m_init,
                           float* reproj, double* resid, double* alpha, const icon_solve_opts* opts);
+
+/* ---- packed solve of several series ------------------------------------ */
+/* A batch packs tilt series of uneven width, thickness, tilt range and slice
+ * count (one plan each, all on one device) into the same launches: every
+ * kernel of the iteration runs once per size class (row length, column grid
+ * and tile width) over all series of that class, and the whole solve is
+ * replayed as one HIP graph.  Replaces the reference's per-series set-up and
+ * slice loop (MainCode.cu:63-71,113-162) for a whole set of series.
+ * slices[i] (<= the plan's max_slices) slices of series i are solved by
+ * every icon_batch_reconstruct.  The plans must outlive the batch; a batch
+ * overwrites their solve buffers. */
+typedef struct icon_batch icon_batch;
+int icon_batch_create(icon_batch** batch, int nseries, icon_plan* const* plans, const int* slices);
+int icon_batch_destroy(icon_batch* batch);
+/* icon_reconstruct for every series at once.  Each argument is an array of
+ * nseries device pointers laid out as for icon_reconstruct on that series'
+ * plan; d_m_init, d_reproj, d_resid and d_alpha may be NULL, or hold NULL
+ * entries.  The options (iteration split, data type, threshold) apply to all
+ * series; reuse_data is not supported.  use_graph is ignored (always a graph).
+ * Asynchronous on `stream`. */
+int icon_batch_reconstruct(icon_batch* batch, const void* const* d_sino, void* const* d_m,
+                           const void* const* d_m_init, void* const* d_reproj, void* const* d_resid,
+                           void* const* d_alpha, const icon_solve_opts* opts, void* stream);
+/* Kernel launches of one iteration of the packed solve. */
+int icon_batch_launches_per_iteration(const icon_batch* batch, int* launches);
+/* Autotuner of the column tile (the spread/interpolation tile and bin width)
+ * per column size class: times the class's packed interpolation and spread
+ * launches for every tile width it supports and keeps the fastest for all
+ * its plans; col_blocks (nseries entries, or NULL) receives the choice per
+ * series.  Overwrites the plans' solve buffers.  No counterpart in the
+ * reference. */
+int icon_batch_autotune(icon_batch* batch, int* col_blocks);
 
 /* ---- host-only helpers (no device needed) -------------------------------- */
 /* The plan geometry computed on the host exactly as icon_plan_create does

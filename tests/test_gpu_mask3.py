@@ -111,5 +111,5 @@ def test_mask3_cli_end_to_end(icon, tmp_path):
     # no mask: crop and stack only; exit status 1 after simpleCatch, as the reference
     r = subprocess.run(base + ["-o", str(tmp_path / "nm.mrc"), "-nm", "1"], capture_output=True, text=True,
                        timeout=120)
-    assert r.returncode == 1 and "simple Catch" in r.stdout, r.stdout + r.stderr
+    assert r.returncode == 1 and "without the Fourier mask" in r.stdout, r.stdout + r.stderr
     np.testing.assert_array_equal(mrc.read(str(tmp_path / "nm.mrc"))[1], mids[:, y0:y0 + th, :])

@@ -1,0 +1,124 @@
+"""The learned warm start end to end on the device (the benchmark's config 5 at
+a bounded size): 512^2 slices, 41 tilts +-60 deg, plain ICON 200 iterations
+(20 INFR, 160 ICON, 20 INFR, m >= 0).
+
+1. (10-iteration, 200-iteration) pairs of a training phantom from the device
+   solver, the U-Net trained on them by its own BF16 kernels.
+2. The continuation length k calibrated on the training phantom: the
+   smallest k that brings every training slice to its plain-ICON residual.
+3. On a separate evaluation phantom, the learned solve (10 plain iterations,
+   the U-Net, k descent iterations) with the per-slice fallback to plain
+   ICON: every slice ends at or below its plain-ICON residual.
+4. What the network gains: the learned result reproduces plain ICON's
+   reconstruction (per-slice relative L2 and FRC against it) far more
+   closely than the same 10 + k iterations without the network, and its FRC
+   against the phantom is no more than 0.005 below plain ICON's.  (At this
+   SNR the descent without the network scores as high against the phantom:
+   ICON's constrained iterations fill the missing wedge with noise-driven
+   content, and the stack's reprojection FRC, a data-fit measure, does not
+   separate the methods either; what the network buys is ICON's result.)
+5. The fallback path: slices given an unreachable target are re-solved by
+   plain ICON and come back equal to the plain solve.
+"""
+import numpy as np
+import pytest
+
+from feilab_amd import phantom, warmstart
+
+pytestmark = pytest.mark.gpu
+
+SCHED = (20, 160, 20)
+START = 10
+
+
+def rel_rows(a, b):
+    a = np.asarray(a, np.float64).reshape(len(a), -1)
+    b = np.asarray(b, np.float64).reshape(len(b), -1)
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+def _plain(icon, plan, d_sino, S):
+    from feilab_amd.icon import DeviceArray, SolveOptions
+    d_m = DeviceArray((S, plan.N0, plan.N1), np.float32)
+    d_r = DeviceArray((S, sum(SCHED) + 1), np.float64)
+    d_rp = DeviceArray((S, plan.N0), np.float32)
+    plan.reconstruct_dev(S, d_sino, d_m, SolveOptions(*SCHED, 2, 0.0, 1, 0), d_resid=d_r, d_reproj=d_rp)
+    icon.lib().icon_device_synchronize()
+    return d_m.numpy(), d_r.numpy()[:, -1], d_rp.numpy()
+
+
+def test_learned_warm_start_end_to_end(icon):
+    from feilab_amd.icon import DeviceArray
+    S_tr, S_ev = 64, 32
+    tr = phantom.config2(nslices=S_tr, seed=2001)
+    print("warm start: %d training pairs (10, 200 iterations) of 512^2 slices" % S_tr, flush=True)
+    plan = icon.Plan(512, tr["tilt"], max_slices=S_tr)
+    x, t = warmstart.make_pairs(plan, [tr["sino"][:32], tr["sino"][32:]], S_tr, START, SCHED)
+    params, losses = warmstart.train_on_pairs(x, t, steps=800)
+    print("trained 800 steps: loss %.4f -> %.4f" % (np.mean(losses[:20]), np.mean(losses[-50:])), flush=True)
+    assert np.mean(losses[-50:]) < 0.8 * np.mean(losses[:20])
+
+    # k from the training phantom
+    d_tr = DeviceArray.from_host(tr["sino"])
+    _, R_tr, _ = _plain(icon, plan, d_tr, S_tr)
+    solver = warmstart.LearnedSolver(plan, params, S_tr, start=START, schedule=SCHED, batch=32)
+    k = warmstart.calibrate_continuation(solver, S_tr, d_tr, R_tr, max_cont=40)
+    print("continuation calibrated on the training phantom: k = %s" % k, flush=True)
+    assert k is not None and k <= 20
+
+    # evaluation phantom: plain ICON, then the learned solve with its fallback
+    ev = phantom.config2(nslices=S_ev)
+    d_ev = DeviceArray.from_host(ev["sino"])
+    m_plain, R_plain, rp_plain = _plain(icon, plan, d_ev, S_ev)
+    d_m = DeviceArray((S_ev, 512, 512), np.float32)
+    d_r = DeviceArray((S_ev, k + 1), np.float64)
+    d_rp = DeviceArray((S_ev, 512), np.float32)
+    bad = solver.solve(S_ev, d_ev, d_m, k, R_plain, d_r, d_rp)
+    m_l, R_l, rp_l = d_m.numpy(), d_r.numpy()[:, -1], d_rp.numpy()
+    print("learned: %d of %d slices fell back; R / target max %.4f" % (len(bad), S_ev, (R_l / R_plain).max()),
+          flush=True)
+    assert (R_l <= R_plain).all()
+
+    # the same iterations without the network
+    d_mn = DeviceArray((S_ev, 512, 512), np.float32)
+    d_rn = DeviceArray((S_ev, k + 1), np.float64)
+    d_rpn = DeviceArray((S_ev, 512), np.float32)
+    solver.run(S_ev, d_ev, d_mn, k, d_resid=d_rn, d_reproj=d_rpn, use_net=False)
+    icon.lib().icon_device_synchronize()
+    m_n, rp_n = d_mn.numpy(), d_rpn.numpy()
+    e_net, e_none = rel_rows(m_l, m_plain), rel_rows(m_n, m_plain)
+    band = (1, 128)
+    _, f_plain = warmstart.reprojection_frc(rp_plain, ev["sino"], plan.iomit, band)
+    _, f_net = warmstart.reprojection_frc(rp_l, ev["sino"], plan.iomit, band)
+    _, f_none = warmstart.reprojection_frc(rp_n, ev["sino"], plan.iomit, band)
+    from feilab_amd.frc import frc_slices
+    truth = phantom.render_slices_3d(ev["blobs"], 512, 512, np.arange(S_ev))
+    q = {name: frc_slices(v, truth)[:, band[0]:band[1]].mean(axis=1)
+         for name, v in (("plain", m_plain), ("learned", m_l), ("none", m_n))}
+    print("rel L2 to plain ICON per slice: with the network mean %.4f max %.4f, without mean %.4f max %.4f"
+          % (e_net.mean(), e_net.max(), e_none.mean(), e_none.max()), flush=True)
+    print("FRC against the phantom (mean of slices): plain %.4f, learned %.4f, without the network %.4f"
+          % (q["plain"].mean(), q["learned"].mean(), q["none"].mean()), flush=True)
+    print("stack reprojection FRC: plain %.5f, learned %.5f, without the network %.5f" % (f_plain, f_net, f_none),
+          flush=True)
+    f_lp = frc_slices(m_l, m_plain)[:, band[0]:band[1]].mean(axis=1)
+    f_np = frc_slices(m_n, m_plain)[:, band[0]:band[1]].mean(axis=1)
+    print("FRC against plain ICON (mean of slices): learned %.4f, without the network %.4f"
+          % (f_lp.mean(), f_np.mean()), flush=True)
+    assert e_net.mean() < 0.7 * e_none.mean()
+    assert f_lp.mean() > f_np.mean() + 0.1
+    assert (q["learned"] >= q["plain"] - 0.005).all()
+
+    # forced fallback: two slices with an unreachable target
+    tgt = R_plain.copy()
+    tgt[[3, 17]] = 0.0
+    bad2 = solver.solve(S_ev, d_ev, d_m, k, tgt, d_r, d_rp)
+    assert {3, 17} <= set(bad2.tolist())
+    m_f, R_f = d_m.numpy(), d_r.numpy()[:, -1]
+    for s in (3, 17):
+        assert rel_rows(m_f[s:s + 1], m_plain[s:s + 1])[0] < 1e-6
+        assert R_f[s] == pytest.approx(R_plain[s], rel=1e-9)
+    keep = [s for s in range(S_ev) if s not in set(bad2.tolist())]
+    np.testing.assert_array_equal(m_f[keep], m_l[keep])
+    solver.close()
+    plan.close()
