@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,42 @@ int perr(int code, const std::string& m) {
   return code;
 }
 
+// Device memory of icon_preprocess_noalloc, in place of the two buffers that
+// icon_preprocess allocates per call: the subtracted modes (double, staged
+// out) at offset 0, the tilt angles (float, staged in) after them.  Each
+// device holds its own copy until the process ends.
+constexpr int kNoAllocMaxImages = 4096;
+constexpr size_t kScratchModes = 0;
+constexpr size_t kScratchTilts = sizeof(double) * kNoAllocMaxImages;
+constexpr size_t kScratchBytes = kScratchTilts + sizeof(float) * kNoAllocMaxImages;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on each device, resolved once per device with
+// that device current (the array's name in host code is only a stand-in).
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;
+
+hipError_t scratch_on_current_device(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -127,6 +164,35 @@ int icon_preprocess(int device, float* d_stack, int nx, int ny, int nz, const fl
   if (d_tilt) (void)hipFree(d_tilt);
   if (d_modes) (void)hipFree(d_modes);
   if (e != hipSuccess) return perr(ICON_ERR_DEVICE, std::string("icon_preprocess: ") + hipGetErrorString(e));
+  return ICON_OK;
+}
+
+int icon_preprocess_noalloc(int device, float* d_stack, int nx, int ny, int nz, const float* tilt_deg, int thickness,
+                            double* modes, void* stream) {
+  if (!d_stack || nx < 1 || ny < 1 || nz < 1) return perr(ICON_ERR_ARG, "icon_preprocess_noalloc: bad argument");
+  if (nz > kNoAllocMaxImages)
+    return perr(ICON_ERR_CAPACITY, "icon_preprocess_noalloc: " + std::to_string(nz) + " images exceed the " +
+                                       std::to_string(kNoAllocMaxImages) + " its device scratch holds per call");
+  if (hipSetDevice(device) != hipSuccess) return perr(ICON_ERR_ARG, "icon_preprocess_noalloc: no such device");
+  unsigned char* base = nullptr;
+  hipError_t e = scratch_on_current_device(&base);
+  if (e != hipSuccess)
+    return perr(ICON_ERR_DEVICE, std::string("icon_preprocess_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+  hipStream_t st = (hipStream_t)stream;
+  const bool norm = tilt_deg && thickness > 0;
+  // every launch writes all nz modes and reads only the nz tilts staged here,
+  // so nothing left in the scratch by an earlier call is read
+  float* d_tilt = norm ? reinterpret_cast<float*>(base + kScratchTilts) : nullptr;
+  double* d_modes = modes ? reinterpret_cast<double*>(base + kScratchModes) : nullptr;
+  if (norm) e = hipMemcpyAsync(d_tilt, tilt_deg, sizeof(float) * nz, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_preprocess, dim3(nz), dim3(kThreads), 0, st, d_stack, nx, ny, d_tilt, norm ? thickness : 0,
+                       d_modes);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && modes) e = hipMemcpyAsync(modes, d_modes, sizeof(double) * nz, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return perr(ICON_ERR_DEVICE, std::string("icon_preprocess_noalloc: ") + hipGetErrorString(e));
   return ICON_OK;
 }
 

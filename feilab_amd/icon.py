@@ -103,6 +103,7 @@ def lib():
                                ctypes.POINTER(ci), ctypes.POINTER(cf), ci],
         "icon_frc": [vp, vp, ci, ci, vp],
         "icon_preprocess": [ci, vp, ci, ci, ci, vp, ci, vp, vp],
+        "icon_preprocess_noalloc": [ci, vp, ci, ci, ci, vp, ci, vp, vp],
         "icon_preprocess_file": [ci, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ci],
         "icon_mask3_weights": [ci, vp, ci, cf, cf, ci, ci, cf, vp],
         "icon_mask3_filter": [ci, vp, ci, ci, vp, vp],
@@ -526,6 +527,20 @@ def frc_smooth(curve, overlap=10):
     c = np.ascontiguousarray(curve, np.float32).copy()
     lib().icon_frc_smooth(c.ctypes.data_as(ctypes.c_void_p), len(c), int(overlap))
     return c
+
+
+def preprocess_dev(d_stack, nz, ny, nx, tilt_deg=None, thickness=0, device=0, noalloc=False):
+    """ICONPreProcess in place on a device stack [nz][ny][nx] (a DeviceArray);
+    returns the subtracted modes [nz].  noalloc: icon_preprocess_noalloc (no
+    device allocation, at most 4096 images)."""
+    modes = np.zeros(nz)
+    tl = None if tilt_deg is None else np.ascontiguousarray(tilt_deg, np.float32)
+    fn = lib().icon_preprocess_noalloc if noalloc else lib().icon_preprocess
+    rc = fn(device, d_stack.ptr, nx, ny, nz, None if tl is None else tl.ctypes.data_as(ctypes.c_void_p),
+            int(thickness), modes.ctypes.data_as(ctypes.c_void_p), None)
+    if rc:
+        raise IconError(rc, lib().icon_preprocess_error().decode(errors="replace"))
+    return modes
 
 
 def preprocess(stack, tilt_deg=None, thickness=0, device=0):

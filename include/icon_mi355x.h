@@ -284,6 +284,16 @@ void icon_frc_smooth(float* frc, int size, int overlap);
  * receives the subtracted modes.  Synchronous. */
 int icon_preprocess(int device, float* d_stack, int nx, int ny, int nz, const float* tilt_deg, int thickness,
                     double* modes, void* stream);
+/* icon_preprocess without any device allocation, for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and
+ * synchronisation as icon_preprocess; the tilt angles and modes are staged in
+ * a fixed device array of the library (48 KiB per device, held until the
+ * process ends) instead of buffers allocated per call, so nz is limited to
+ * 4096 images per call (ICON_ERR_CAPACITY beyond).  Calls that overlap on one
+ * device share that array and race; calls on different devices do not. */
+int icon_preprocess_noalloc(int device, float* d_stack, int nx, int ny, int nz, const float* tilt_deg, int thickness,
+                            double* modes, void* stream);
 /* The same on MRC files: in_path -> out_path (float, header copied from the
  * input, statistics recomputed); tilt_path NULL or thickness <= 0: mode
  * subtraction only. */
