@@ -254,13 +254,15 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
   }
 }
 
+// At most 2 waves per SIMD requested (132 registers): capped at 128 the fused
+// forward pass spilled, and the kernel ran 2 % slower.
 template <int NR, int RPB, bool FWD>
-__global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj(RowAdjArgs a) {
+__global__ __launch_bounds__(RPB * (NR / 16), 2) void k_row_adj(RowAdjArgs a) {
   row_adj_body<NR, RPB, FWD>(a, blockIdx.x);
 }
 
 template <int NR, int RPB, bool FWD>
-__global__ __launch_bounds__(RPB * (NR / 16), 4) void k_row_adj_multi(const RowAdjArgs* __restrict__ descs,
+__global__ __launch_bounds__(RPB * (NR / 16), 2) void k_row_adj_multi(const RowAdjArgs* __restrict__ descs,
                                                                      const int2* __restrict__ items, int mode) {
   const int2 it = items[blockIdx.x];
   RowAdjArgs a = descs[it.x];
@@ -326,19 +328,13 @@ __device__ __forceinline__ void col_fwd_body(const ColArgs& a, int bid) {
   constexpr int NT = C * (NC / 16);
   constexpr int LS = col_stride<NC, C>();
   extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
-  // Block order.  A tile's grid rows are 8 C bytes of each 128-byte line, so
-  // the tiles 2p and 2p + 1 of a slice read the two halves of the same lines:
-  // they run as blocks b and b + 8, which the dispatcher places on the same
-  // XCD (workgroups go round-robin over the 8 XCDs) one after the other, so
-  // the second half comes from that XCD's L2 instead of a second HBM read.
-  // Each group of 16 blocks takes 8 consecutive (slice, tile pair) items;
-  // items are tile-pair-major, so blocks in flight together also share the
-  // pair's bin records in L2.  S * n1 / C is a multiple of 16 (n1 / C is a
-  // power of two >= 16).
-  const int item = (bid >> 4) * 8 + (bid & 7);
-  const int pair = item / a.S;
-  const int s = item % a.S;
-  const int blk = 2 * pair + ((bid >> 3) & 1);
+  // tile-major order: blocks in flight together work on the same column
+  // tile of different slices and share its bin records in L2 (an order that
+  // put the two tiles sharing each 128-byte grid line on one XCD back to back
+  // halved the L2 misses but ran 3 % slower: the repeated lines come from
+  // the memory-side cache)
+  const int blk = bid / a.S;
+  const int s = bid % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
   const int h0 = a.N0 >> 1;
