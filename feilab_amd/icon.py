@@ -116,6 +116,8 @@ def lib():
         "icon_mrc_write_slice": [ctypes.c_char_p, ctypes.c_char_p, vp, ci, ci],
         "icon_time_kernel": [vp, ci, ci, ci, ctypes.POINTER(ctypes.c_double)],
         "icon_copy_probe": [vp, vp, cs, ci, ctypes.POINTER(ctypes.c_double)],
+        "icon_reconstruct_fp64": [ci, ci, ci, ci, ctypes.POINTER(cf), ci, vp, vp, vp, vp, vp,
+                                  ctypes.POINTER(SolveOptions)],
         "icon_batch_create": [ctypes.POINTER(vp), ci, vp, vp],
         "icon_batch_destroy": [vp],
         "icon_batch_reconstruct": [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolveOptions), vp],
@@ -345,6 +347,29 @@ class Plan:
             out.append(rs[0] if single else rs)
             out.append((al[0] if single else al)[..., :iters])
         return out[0] if len(out) == 1 else tuple(out)
+
+
+def reconstruct_fp64(nx, tilt_deg, sino, pre, icon, post, data_type=2, threshold=0.0, thickness=0, device=0):
+    """The all-FP64 device solve (icon_reconstruct_fp64) of sinograms [S][nz][nx]:
+    (images [S][nx][thickness], reprojections [S][nx], residual traces
+    [S][iters+1], step sizes [S][iters]), all float64."""
+    tilt = np.ascontiguousarray(tilt_deg, np.float32)
+    sino = np.ascontiguousarray(sino, np.float32)
+    if sino.ndim == 2:
+        sino = sino[None]
+    S, nz = sino.shape[0], len(tilt)
+    N1 = thickness or nx
+    iters = pre + icon + post
+    m = np.zeros((S, nx, N1))
+    rp = np.zeros((S, nx))
+    rs = np.zeros((S, iters + 1))
+    al = np.zeros((S, max(iters, 1)))
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    _check(lib().icon_reconstruct_fp64(device, int(nx), int(thickness), nz,
+                                       tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), S, P(sino), P(m), P(rp),
+                                       P(rs), P(al), ctypes.byref(SolveOptions(pre, icon, post, data_type, threshold,
+                                                                               0, 0))), "icon_reconstruct_fp64")
+    return m, rp, rs, al[:, :iters]
 
 
 class Batch:

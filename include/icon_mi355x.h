@@ -168,6 +168,17 @@ int icon_reconstruct(icon_plan* plan, int S, const void* d_sino, void* d_m, cons
                      void* d_reproj, void* d_resid, void* d_alpha, const icon_solve_opts* opts,
                      void* stream);
 
+/* The same solve entirely in double precision on the device (the reference's
+ * double build, DATATYPE = double, nufft_gpu_v8.cuh:25-33): FP64 window taps
+ * and deapodisation (sigma = n/N), FP64 FFTs, spread and interpolation, FP64
+ * reductions and update; the Gaussian window.  A precision reference for the
+ * FP32 solve, not a fast path.  Host buffers, synchronous, device memory
+ * allocated per call: sino [S][nz][nx] float in; m [S][nx][thickness],
+ * reproj [S][nx] (or NULL), resid [S][iters+1] (or NULL) and alpha
+ * [S][iters] (or NULL) double out. */
+int icon_reconstruct_fp64(int device, int nx, int thickness, int nz, const float* tilt_deg, int S, const float* sino,
+                          double* m, double* reproj, double* resid, double* alpha, const icon_solve_opts* opts);
+
 /* Host-buffer convenience wrapper of icon_reconstruct (synchronous). */
 int icon_reconstruct_host(icon_plan* plan, int S, const float* sino, float* m, const float* m_init,
                           float* reproj, double* resid, double* alpha, const icon_solve_opts* opts);
