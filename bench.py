@@ -17,9 +17,9 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   plain ICON re-solving every slice whose R misses its target (the per-slice
   fallback, timed as part of the step).  k is calibrated once on a held-out
   phantom of the training distribution, never on the evaluated series: the
-  smallest k at which every calibration slice reaches its plain-ICON
-  residual with an FRC against the phantom no more than 0.005 below plain
-  ICON's.
+  smallest k at which every calibration slice reaches 99 % of its plain-ICON
+  residual (the margin keeps fallbacks on the evaluated series rare) with an
+  FRC against the phantom no more than 0.005 below plain ICON's.
 * Quality (not timed): per slice, relative L2 and FRC of the learned result
   against plain ICON's and against the phantom, and the same for the 10 + k
   iterations without the network.
@@ -298,7 +298,8 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
     params, losses = warmstart.train_on_pairs(x, t, steps=args.train_steps, log=log)
     train_s = time.perf_counter() - t0
     del x, t
-    solver = warmstart.LearnedSolver(plan, params, S, start=args.start, schedule=sched, batch=args.unet_batch)
+    solver = warmstart.LearnedSolver(plan, params, S, start=args.start, schedule=sched, batch=args.unet_batch,
+                                     overlap=bool(args.overlap))
 
     # continuation length from a held-out phantom of the training distribution
     Sc = args.calib_slices
@@ -322,10 +323,11 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
                    frc_truth_mean=round(float(qk.mean()), 4))
         scan.append(row)
         log("calibration k=%d:" % k, json.dumps(row))
-        if (Rk <= R_c).all() and (qk >= q_plain_c - args.frc_tol).all():
+        if (Rk <= (1.0 - args.calib_margin) * R_c).all() and (qk >= q_plain_c - args.frc_tol).all():
             k_star = k
             break
-    calib = dict(phantom_seed=3001, slices=Sc, frc_truth_plain_mean=round(float(q_plain_c.mean()), 4), scan=scan)
+    calib = dict(phantom_seed=3001, slices=Sc, resid_margin=args.calib_margin,
+                 frc_truth_plain_mean=round(float(q_plain_c.mean()), 4), scan=scan)
     for a in (d_c, d_mc, d_rc):
         a.free()
     if k_star is None:
@@ -348,6 +350,16 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
     ms_step = dt_l / args.steps * 1e3
     R_l = d_rl.numpy()[:, -1]
     m_l = d_ml.numpy()
+    serial = None
+    if solver.overlap:
+        # the same step with the two halves one after the other: same results, its time
+        solver.overlap = False
+        dt_s = timed(learned_step, "learned, halves not overlapped")
+        same = bool(np.array_equal(d_ml.numpy(), m_l))
+        solver.overlap = True
+        serial = dict(slices_per_s=round(S * args.steps / dt_s, 3), ms_per_step=round(dt_s / args.steps * 1e3, 3),
+                      same_result=same)
+        log("serial learned step:", json.dumps(serial))
     _, frc_l = warmstart.reprojection_frc(d_rpl.numpy(), cfg["sino"], plan.iomit, band)
 
     # quality: against plain ICON and against the phantom; the same iterations without the network
@@ -383,7 +395,8 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
                    quality=quality, calibration=calib, train_pairs=args.train_pairs, train_steps=args.train_steps,
                    train_loss_first=round(float(np.mean(losses[:20])), 4),
                    train_loss_last=round(float(np.mean(losses[-50:])), 4),
-                   pairs_seconds=round(pairs_s, 1), train_seconds=round(train_s, 1), unet_batch=args.unet_batch)
+                   pairs_seconds=round(pairs_s, 1), train_seconds=round(train_s, 1), unet_batch=args.unet_batch,
+                   overlapped_halves=bool(solver.overlap), without_overlap=serial)
     log("learned:", json.dumps(learned))
 
     # the U-Net forward alone
@@ -415,6 +428,10 @@ def main():
     ap.add_argument("--pair-chunk", type=int, default=256, help="slices per training phantom")
     ap.add_argument("--calib-slices", type=int, default=64, help="slices of the held-out calibration phantom")
     ap.add_argument("--unet-batch", type=int, default=128, help="slices per U-Net forward launch sequence")
+    ap.add_argument("--overlap", type=int, default=0,
+                    help="1: the learned solve in two halves on two streams (iterations overlap the U-Net)")
+    ap.add_argument("--calib-margin", type=float, default=0.01,
+                    help="calibration target: R <= (1 - margin) x plain ICON's R on every calibration slice")
     ap.add_argument("--max-cont", type=int, default=40, help="continuation iterations tried")
     ap.add_argument("--frc-tol", type=float, default=0.005,
                     help="allowed drop of the FRC against the phantom below plain ICON's")
@@ -563,8 +580,9 @@ def main():
         log("cpu baseline:", json.dumps(cpu))
     workload = ("learned-start ICON vs plain ICON: %d slices of 512x512, 41 tilts +-60 deg step 3, SNR 0.05; "
                 "target = per-slice data residual of plain ICON (%d iterations %d,%d,%d, m >= 0), the continuation "
-                "length calibrated on a held-out training-distribution phantom where every slice also keeps its FRC "
-                "against the phantom within %.3f of plain ICON's; U-Net trained on %d (%d, %d)-iteration pairs; "
+                "length calibrated on a held-out training-distribution phantom where every slice reaches 99%% of its "
+                "plain-ICON residual and keeps its FRC against the phantom within %.3f of plain ICON's; U-Net "
+                "trained on %d (%d, %d)-iteration pairs; "
                 "per-slice fallback to plain ICON timed in the step; sinograms resident in HBM, solves replayed as "
                 "HIP graphs" % (S, iters, pre, icon_it, post, args.frc_tol, args.train_pairs, args.start, iters))
     out = {

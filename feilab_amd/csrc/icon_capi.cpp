@@ -367,6 +367,49 @@ int icon_get_device(int* device) {
   return ICON_OK;
 }
 
+int icon_stream_create(void** stream) {
+  if (!stream) return fail(ICON_ERR_ARG, "icon_stream_create: null pointer");
+  hipStream_t st = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *stream = st;
+  return ICON_OK;
+}
+
+int icon_stream_destroy(void* stream) {
+  if (stream) HIPCHK(hipStreamDestroy((hipStream_t)stream));
+  return ICON_OK;
+}
+
+int icon_stream_synchronize(void* stream) {
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return ICON_OK;
+}
+
+int icon_event_create(void** event) {
+  if (!event) return fail(ICON_ERR_ARG, "icon_event_create: null pointer");
+  hipEvent_t ev = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  *event = ev;
+  return ICON_OK;
+}
+
+int icon_event_destroy(void* event) {
+  if (event) HIPCHK(hipEventDestroy((hipEvent_t)event));
+  return ICON_OK;
+}
+
+int icon_event_record(void* event, void* stream) {
+  if (!event) return fail(ICON_ERR_ARG, "icon_event_record: null event");
+  HIPCHK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return ICON_OK;
+}
+
+int icon_stream_wait_event(void* stream, void* event) {
+  if (!event) return fail(ICON_ERR_ARG, "icon_stream_wait_event: null event");
+  HIPCHK(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+  return ICON_OK;
+}
+
 int icon_malloc(void** dptr, size_t bytes) {
   if (!dptr) return fail(ICON_ERR_ARG, "icon_malloc: null pointer");
   HIPCHK(hipMalloc(dptr, bytes ? bytes : 1));

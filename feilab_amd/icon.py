@@ -81,6 +81,13 @@ def lib():
         "icon_memcpy_d2h": [vp, vp, cs],
         "icon_memset_d": [vp, ci, cs],
         "icon_memcpy_d2d_async": [vp, vp, cs, vp],
+        "icon_stream_create": [ctypes.POINTER(vp)],
+        "icon_stream_destroy": [vp],
+        "icon_stream_synchronize": [vp],
+        "icon_event_create": [ctypes.POINTER(vp)],
+        "icon_event_destroy": [vp],
+        "icon_event_record": [vp, vp],
+        "icon_stream_wait_event": [vp, vp],
         "icon_device_synchronize": [],
         "icon_plan_create": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
         "icon_plan_create_cv": [ctypes.POINTER(vp), ci, ci, ci, ci, ctypes.POINTER(cf), ci],
@@ -199,6 +206,55 @@ class DeviceArray:
             pass
 
 
+class Stream:
+    """A HIP stream of the current device (default priority)."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        _check(lib().icon_stream_create(ctypes.byref(self.h)), "icon_stream_create")
+
+    @property
+    def value(self):
+        return self.h.value
+
+    def synchronize(self):
+        _check(lib().icon_stream_synchronize(self.h), "icon_stream_synchronize")
+
+    def record(self, event):
+        _check(lib().icon_event_record(event.h, self.h), "icon_event_record")
+
+    def wait(self, event):
+        _check(lib().icon_stream_wait_event(self.h, event.h), "icon_stream_wait_event")
+
+    def close(self):
+        if self.h and self.h.value and _lib is not None:
+            _lib.icon_stream_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        _check(lib().icon_event_create(ctypes.byref(self.h)), "icon_event_create")
+
+    def close(self):
+        if self.h and self.h.value and _lib is not None:
+            _lib.icon_event_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def copy_d2d(dst, src, nbytes, stream=None):
     """Asynchronous device-to-device copy between device addresses (ints or DeviceArrays)."""
     _check(lib().icon_memcpy_d2d_async(_ptr(dst), _ptr(src), int(nbytes), _ptr(stream)), "icon_memcpy_d2d_async")
@@ -209,6 +265,8 @@ def _ptr(x):
         return None
     if isinstance(x, DeviceArray):
         return x.ptr
+    if isinstance(x, Stream):
+        return x.h
     return ctypes.c_void_p(int(x))
 
 
@@ -227,6 +285,7 @@ class Plan:
         window: "gauss" (the reference's 14-tap Gaussian) or "kb6" (6-tap
         Kaiser-Bessel)."""
         tilt = np.ascontiguousarray(tilt_deg, np.float32)
+        self.tilt, self.thickness, self.window, self.device = tilt.copy(), int(thickness), window, device
         self.h = ctypes.c_void_p()
         tp = tilt.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         if window == "gauss":

@@ -48,6 +48,14 @@ def _p(x):
     return x.ptr if isinstance(x, DeviceArray) else ctypes.c_void_p(int(x))
 
 
+def _s(stream):
+    """A stream argument: None (the default stream), a Stream or a raw handle."""
+    if stream is None:
+        return None
+    h = getattr(stream, "h", None)
+    return h if h is not None else ctypes.c_void_p(int(stream))
+
+
 class UNet:
     """U-Net on one device for batches of up to max_batch images of H x W."""
 
@@ -83,7 +91,7 @@ class UNet:
         _check(_lib().icon_unet_set_params(self.h, p.ctypes.data_as(ctypes.c_void_p), self.nparam), "set_params")
 
     def forward_dev(self, B, d_in, d_out, stream=None):
-        _check(_lib().icon_unet_forward(self.h, B, _p(d_in), _p(d_out), stream), "forward")
+        _check(_lib().icon_unet_forward(self.h, B, _p(d_in), _p(d_out), _s(stream)), "forward")
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)
@@ -95,7 +103,7 @@ class UNet:
 
     def warm_start_dev(self, S, d_m, d_out, stream=None):
         """d_out = m + s * U(m / s) per slice, s = RMS of the slice (device buffers)."""
-        _check(_lib().icon_unet_warm_start(self.h, S, _p(d_m), _p(d_out), stream), "warm_start")
+        _check(_lib().icon_unet_warm_start(self.h, S, _p(d_m), _p(d_out), _s(stream)), "warm_start")
 
     def warm_start(self, m):
         m = np.ascontiguousarray(m, np.float32)

@@ -22,7 +22,7 @@ import math
 
 import numpy as np
 
-from .icon import DeviceArray, SolveOptions, copy_d2d, lib
+from .icon import DeviceArray, Event, Plan, SolveOptions, Stream, copy_d2d, lib
 from .unet import UNet
 
 
@@ -161,13 +161,23 @@ def train_warm_start(plan, sino, start, schedule, steps=1500, patch=128, batch=1
 
 class LearnedSolver:
     """Learned-start reconstruction of up to max_slices slices on a plan, with
-    plain ICON (schedule = (pre, icon, post)) as the per-slice fallback."""
+    plain ICON (schedule = (pre, icon, post)) as the per-slice fallback.
+
+    overlap=True splits the slices into two halves solved on two streams, each
+    with its own plan and network: the second half starts its plain
+    iterations when the first half's are done, so they run while the first
+    half's U-Net runs (the iterations are VALU- and LDS-bound, the U-Net
+    matrix-core-bound), and the first half's continuation overlaps the
+    second half's U-Net.  Results are those of the serial run: every slice
+    goes through the same kernels."""
 
     def __init__(self, plan, params, max_slices, start=10, schedule=(20, 160, 20), data_type=2, threshold=0.0,
-                 device=0, batch=128):
+                 device=0, batch=128, overlap=False):
         self.plan, self.start, self.data_type, self.threshold = plan, start, data_type, threshold
         self.schedule = tuple(int(v) for v in schedule)
-        self.net = UNet(plan.N0, plan.N1, max_batch=min(batch, max_slices), device=device)
+        self.overlap = bool(overlap) and max_slices >= 2
+        half = (max_slices + 1) // 2
+        self.net = UNet(plan.N0, plan.N1, max_batch=min(batch, half if self.overlap else max_slices), device=device)
         self.net.set_params(params)
         shape = (max_slices, plan.N0, plan.N1)
         self.d_start = DeviceArray(shape, np.float32)
@@ -177,20 +187,62 @@ class LearnedSolver:
         self.d_fb_m = DeviceArray(shape, np.float32)
         self.d_fb_rp = DeviceArray((max_slices, plan.N0), np.float32)
         self.d_fb_res = DeviceArray((max_slices, iters + 1), np.float64)
+        if self.overlap:
+            self.plan2 = Plan(plan.N0, plan.tilt, thickness=plan.thickness, max_slices=half, device=device,
+                              cross_validation=plan.cross_validation, window=plan.window)
+            if plan.info.col_block != self.plan2.info.col_block:
+                self.plan2.set_col_block(plan.info.col_block)
+            self.net2 = UNet(plan.N0, plan.N1, max_batch=min(batch, half), device=device)
+            self.net2.set_params(params)
+            self.streams = (Stream(), Stream())
+            self.started = Event()
+
+    def _chain(self, plan, net, S, d_sino, d_start, d_init, d_m, cont, d_resid, d_reproj, stream, use_net,
+               after_start=None):
+        plan.reconstruct_dev(S, d_sino, d_start, SolveOptions(self.start, 0, 0, self.data_type, self.threshold, 1, 0),
+                             stream=stream)
+        if after_start is not None:
+            after_start()
+        if use_net:
+            net.warm_start_dev(S, d_start, d_init, stream)
+        plan.reconstruct_dev(S, d_sino, d_m, SolveOptions(cont, 0, 0, self.data_type, self.threshold, 1, 1),
+                             d_m_init=d_init if use_net else d_start, d_resid=d_resid, d_reproj=d_reproj,
+                             stream=stream)
 
     def run(self, S, d_sino, d_m, cont, d_resid=None, d_reproj=None, stream=None, use_net=True):
         """start plain iterations -> U-Net -> `cont` descent iterations into d_m;
         d_resid [S][cont + 1] receives R(m) along the continuation, d_reproj
         [S][N0] the reprojections of the result.  use_net=False continues from
         the start iterate itself (the same method without the network)."""
+        if not self.overlap or S < 2:
+            self._chain(self.plan, self.net, S, d_sino, self.d_start, self.d_init, d_m, cont, d_resid, d_reproj,
+                        stream, use_net)
+            return
         p = self.plan
-        p.reconstruct_dev(S, d_sino, self.d_start, SolveOptions(self.start, 0, 0, self.data_type, self.threshold, 1, 0),
-                          stream=stream)
-        if use_net:
-            self.net.warm_start_dev(S, self.d_start, self.d_init, stream)
-        p.reconstruct_dev(S, d_sino, d_m, SolveOptions(cont, 0, 0, self.data_type, self.threshold, 1, 1),
-                          d_m_init=self.d_init if use_net else self.d_start, d_resid=d_resid, d_reproj=d_reproj,
-                          stream=stream)
+        S0 = (S + 1) // 2
+        S1 = S - S0
+        sino_b, img_b, res_b, rp_b = p.nz * p.N0 * 4, p.N0 * p.N1 * 4, (cont + 1) * 8, p.N0 * 4
+        at = lambda a, off: None if a is None else a.addr(off)
+        st0, st1 = self.streams
+        if stream is not None:
+            ev = Event()
+            lib().icon_event_record(ev.h, _stream_handle(stream))
+            st0.wait(ev)
+            st1.wait(ev)
+        self._chain(p, self.net, S0, d_sino, self.d_start, self.d_init, d_m, cont, d_resid, d_reproj, st0, use_net,
+                    after_start=lambda: st0.record(self.started))
+        st1.wait(self.started)
+        self._chain(self.plan2, self.net2, S1, at(d_sino, S0 * sino_b), at(self.d_start, S0 * img_b),
+                    at(self.d_init, S0 * img_b), at(d_m, S0 * img_b), cont, at(d_resid, S0 * res_b),
+                    at(d_reproj, S0 * rp_b), st1, use_net)
+        if stream is None:
+            st0.synchronize()
+            st1.synchronize()
+        else:
+            for st in (st0, st1):
+                ev = Event()
+                st.record(ev)
+                lib().icon_stream_wait_event(_stream_handle(stream), ev.h)
 
     def solve(self, S, d_sino, d_m, cont, target, d_resid, d_reproj=None):
         """The learned solve with its fallback: run(), then every slice whose
@@ -212,8 +264,9 @@ class LearnedSolver:
         row, img, line = p.nz * p.N0 * 4, p.N0 * p.N1 * 4, p.N0 * 4
         for i, s in enumerate(bad):
             copy_d2d(self.d_fb_sino.addr(i * row), d_sino.addr(int(s) * row), row)
+        # replayed as a graph: one capture per distinct count of fallback slices
         p.reconstruct_dev(len(bad), self.d_fb_sino, self.d_fb_m,
-                          SolveOptions(pre, icon_it, post, self.data_type, self.threshold, 0, 0),
+                          SolveOptions(pre, icon_it, post, self.data_type, self.threshold, 1, 0),
                           d_reproj=self.d_fb_rp if d_reproj is not None else None, d_resid=self.d_fb_res)
         for i, s in enumerate(bad):
             s = int(s)
@@ -226,13 +279,25 @@ class LearnedSolver:
         self.net.close()
         for a in (self.d_start, self.d_init, self.d_fb_sino, self.d_fb_m, self.d_fb_rp, self.d_fb_res):
             a.free()
+        if self.overlap:
+            self.net2.close()
+            self.plan2.close()
+            for st in self.streams:
+                st.close()
+            self.started.close()
+
+
+def _stream_handle(stream):
+    h = getattr(stream, "h", None)
+    return h if h is not None else stream
 
 
 def calibrate_continuation(solver, S, d_sino, target, frc_target=None, sino=None, frc_tol=0.005, max_cont=40,
-                           band=None):
+                           band=None, margin=0.0):
     """Smallest continuation k for which every slice of a calibration batch
-    reaches R <= target[s] (and, given frc_target, the stack's reprojection
-    FRC over `band` lies within frc_tol of it); None if no k <= max_cont does."""
+    reaches R <= (1 - margin) target[s] (and, given frc_target, the stack's
+    reprojection FRC over `band` lies within frc_tol of it); None if no
+    k <= max_cont does."""
     p = solver.plan
     d_r = DeviceArray((S, max_cont + 1), np.float64)
     d_rp = DeviceArray((S, p.N0), np.float32)
@@ -242,7 +307,7 @@ def calibrate_continuation(solver, S, d_sino, target, frc_target=None, sino=None
             solver.run(S, d_sino, d_m, k, d_resid=d_r, d_reproj=d_rp)
             lib().icon_device_synchronize()
             R = d_r.numpy().reshape(-1)[:S * (k + 1)].reshape(S, k + 1)[:, k]
-            if not (R <= np.asarray(target)[:S]).all():
+            if not (R <= (1.0 - margin) * np.asarray(target)[:S]).all():
                 continue
             if frc_target is not None:
                 _, f = reprojection_frc(d_rp.numpy(), sino, p.iomit, band)

@@ -90,6 +90,15 @@ int icon_device_count(int* count);
 /* The calling thread's current device (thin HIP wrappers). */
 int icon_set_device(int device);
 int icon_get_device(int* device);
+/* Streams and events on the current device (thin HIP wrappers; streams at
+ * the default priority): for callers that overlap independent solves. */
+int icon_stream_create(void** stream);
+int icon_stream_destroy(void* stream);
+int icon_stream_synchronize(void* stream);
+int icon_event_create(void** event);
+int icon_event_destroy(void* event);
+int icon_event_record(void* event, void* stream);
+int icon_stream_wait_event(void* stream, void* event);
 /* Device memory helpers on the current device (thin HIP wrappers). */
 int icon_malloc(void** dptr, size_t bytes);
 int icon_free(void* dptr);

@@ -120,5 +120,17 @@ def test_learned_warm_start_end_to_end(icon):
         assert R_f[s] == pytest.approx(R_plain[s], rel=1e-9)
     keep = [s for s in range(S_ev) if s not in set(bad2.tolist())]
     np.testing.assert_array_equal(m_f[keep], m_l[keep])
+
+    # the overlapped schedule (two halves on two streams) gives the same result
+    solver2 = warmstart.LearnedSolver(plan, params, S_ev, start=START, schedule=SCHED, batch=32, overlap=True)
+    d_m2 = DeviceArray((S_ev, 512, 512), np.float32)
+    d_r2 = DeviceArray((S_ev, k + 1), np.float64)
+    solver.run(S_ev, d_ev, d_m, k, d_resid=d_r)
+    icon.lib().icon_device_synchronize()
+    solver2.run(S_ev, d_ev, d_m2, k, d_resid=d_r2)
+    np.testing.assert_array_equal(d_m2.numpy(), d_m.numpy())
+    np.testing.assert_array_equal(d_r2.numpy(), d_r.numpy())
+    print("overlapped halves: results bitwise equal to the serial run", flush=True)
+    solver2.close()
     solver.close()
     plan.close()
