@@ -415,6 +415,74 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
     return learned, unet_roofline, (value, ms_step)
 
 
+def config3_bench(args):
+    """The uneven batch (config 3): all 48 series of the generator in one
+    packed solve against the same solves one series (one graph) at a time;
+    prints one JSON line.  The slice counts can be scaled down (--c3-scale)
+    to fit smaller devices; the packed and per-series results are compared."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    import feilab_amd
+    from feilab_amd import phantom
+    from feilab_amd.icon import DeviceArray, SolveOptions
+    L = feilab_amd.lib()
+    specs = phantom.config3_specs()
+    for sp in specs:
+        sp["S"] = max(1, int(round(sp["S"] * args.c3_scale)))
+    pre, icon_it, post = [int(v) for v in args.c3_iters.split(",")]
+    iters = pre + icon_it + post
+    log("config 3: %d series, %d slices, widths %d..%d, %d iterations"
+        % (len(specs), sum(sp["S"] for sp in specs), min(sp["N"] for sp in specs), max(sp["N"] for sp in specs), iters))
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=max(1, min(8, (os.cpu_count() or 2) - 1)),
+                             mp_context=multiprocessing.get_context("spawn")) as ex:
+        sinos = list(ex.map(phantom.config3_sino, specs))
+    log("sinograms ready in %.1f s" % (time.perf_counter() - t0))
+    plans = [feilab_amd.Plan(sp["N"], sp["tilt"], max_slices=sp["S"]) for sp in specs]
+    dev_bytes = sum(p.info.device_bytes for p in plans)
+    d_s = [DeviceArray.from_host(x) for x in sinos]
+    d_m = [DeviceArray((sp["S"], sp["N"], sp["N"]), np.float32) for sp in specs]
+    d_r = [DeviceArray((sp["S"], iters + 1), np.float64) for sp in specs]
+    log("plans: %.1f GB of device memory" % (dev_bytes / 1e9))
+    batch = feilab_amd.Batch(plans, [sp["S"] for sp in specs])
+    cols = batch.autotune()
+    opts = SolveOptions(pre, icon_it, post, 2, 0.0, 1, 0)
+    total = sum(sp["S"] for sp in specs)
+
+    def timed(step, what):
+        for _ in range(args.warmup):
+            step()
+        L.icon_device_synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        L.icon_device_synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        log("%s: %.3f s per solve" % (what, dt))
+        return dt
+
+    dt_packed = timed(lambda: batch.reconstruct_dev(d_s, d_m, opts, d_resids=d_r), "packed")
+    m_packed = [d.numpy() for d in d_m[:6]]
+    r_packed = [d.numpy() for d in d_r]
+
+    def per_series():
+        for i, p in enumerate(plans):
+            p.reconstruct_dev(specs[i]["S"], d_s[i], d_m[i], opts, d_resid=d_r[i])
+
+    dt_series = timed(per_series, "one series at a time")
+    same = all(np.array_equal(d.numpy(), m) for d, m in zip(d_m[:6], m_packed)) and \
+        all(np.array_equal(d.numpy(), r) for d, r in zip(d_r, r_packed))
+    classes = sorted({(p.n0, c) for p, c in zip(plans, cols)})
+    out = dict(metric="config-3 slices/s, all series in packed launches vs one series at a time", unit="slices/s",
+               value=round(total / dt_packed, 2), per_series_slices_per_s=round(total / dt_series, 2),
+               speedup=round(dt_series / dt_packed, 3), same_result=bool(same), series=len(specs), slices=total,
+               iterations=iters, schedule=[pre, icon_it, post], launches_per_iteration=batch.launches_per_iteration(),
+               launches_per_iteration_one_at_a_time=7 * len(specs), grid_and_tile_classes=classes,
+               device_gb=round(dev_bytes / 1e9, 1), slice_scale=args.c3_scale, steps=args.steps, warmup=args.warmup,
+               resid_final_mean=float(np.mean([r[:, -1].mean() for r in r_packed])))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -444,9 +512,19 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-kb", action="store_true", help="skip the Kaiser-Bessel window variant")
     ap.add_argument("--probe", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--workload", default="config2", choices=("config2", "config3"),
+                    help="config3: the uneven batch of 48 series, packed vs one series at a time")
+    ap.add_argument("--c3-iters", default="5,10,5", help="config3: pre,icon,post")
+    ap.add_argument("--c3-scale", type=float, default=1.0, help="config3: factor on every series' slice count")
     args = ap.parse_args()
     if args.probe:
         probe(args.probe, args.slices)
+        return
+    if args.workload == "config3":
+        if args.gpus != 1:
+            print("bench.py measures one GPU (--gpus %d is not supported)" % args.gpus)
+            sys.exit(2)
+        config3_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the ICON solve shards by slice with no exchange between devices, "
