@@ -5,13 +5,14 @@
 // the layout [tap][c_out][c_in] (c_in fastest), taps ky*3 + kx.
 //
 // The 3x3 (and 1x1) convolution is an implicit GEMM on
-// v_mfma_f32_16x16x32_bf16: M = pixels, N = output channels, K = taps x input
-// channels in groups of 32.  Lane l feeds A[pixel l%16][8 channels
-// 8*(l/16)...] and B[8 channels 8*(l/16)...][output channel l%16] as 16-byte
-// vectors from LDS-staged tiles and receives D[pixel 4*(l/16)+i][channel
-// l%16].  The same kernel computes the input gradient with the spatially
-// flipped, channel-transposed weights; the weight gradient is a split-K GEMM
-// over pixels whose operands are transposed through LDS.
+// v_mfma_f32_16x16x32_bf16: M = output channels, N = pixels, K = taps x
+// input channels in groups of 32.  Lane l feeds A[channel slot l%16][8 input
+// channels 8*(l/16)...] (weights) and B[8 input channels 8*(l/16)...][pixel
+// l%16] as 16-byte vectors from LDS-staged tiles and receives D[slots
+// 4*(l/16)+i][pixel l%16], the slots ordered so that they are consecutive
+// output channels.  The same kernel computes the input gradient with the
+// spatially flipped, channel-transposed weights; the weight gradient is a
+// split-K GEMM over pixels whose operands are transposed through LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -33,13 +34,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
-// Barrier of one wavefront for data it exchanges through LDS.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 union V8 {
   uint4 u;
   uint16_t h[8];
@@ -58,31 +52,6 @@ __device__ __forceinline__ float head_group(const V8& v, const float* w, float s
 }
 
 
-// 8 consecutive channels of one pixel as a BF16 vector (zeros if !ok).
-template <typename T>
-__device__ __forceinline__ bf8 load8(const T* p, bool ok);
-
-template <>
-__device__ __forceinline__ bf8 load8<uint16_t>(const uint16_t* p, bool ok) {
-  V8 r;
-  r.u = ok ? *reinterpret_cast<const uint4*>(p) : make_uint4(0, 0, 0, 0);
-  return r.v;
-}
-
-template <>
-__device__ __forceinline__ bf8 load8<float>(const float* p, bool ok) {
-  V8 r;
-  if (ok) {
-    const float4 a = reinterpret_cast<const float4*>(p)[0];
-    const float4 b = reinterpret_cast<const float4*>(p)[1];
-    r.h[0] = f2bf(a.x); r.h[1] = f2bf(a.y); r.h[2] = f2bf(a.z); r.h[3] = f2bf(a.w);
-    r.h[4] = f2bf(b.x); r.h[5] = f2bf(b.y); r.h[6] = f2bf(b.z); r.h[7] = f2bf(b.w);
-  } else {
-    r.u = make_uint4(0, 0, 0, 0);
-  }
-  return r.v;
-}
-
 __device__ __forceinline__ void store1(uint16_t* p, float v) { *p = f2bf(v); }
 __device__ __forceinline__ float load1(const uint16_t* p) { return bf2f(*p); }
 __device__ __forceinline__ float load1(const float* p) { return *p; }
@@ -93,109 +62,183 @@ __device__ __forceinline__ void store1(float* p, float v) { *p = v; }
 // output channels.  Per 32-channel chunk of the input the block stages the
 // tile with its one-pixel halo (18 x 18 pixels) and the chunk's weights for
 // all taps in LDS (rows padded to 80 bytes, so the 16 rows a 16-lane group
-// reads with ds_read_b128 fall in distinct banks); wave w then multiplies
-// tile rows 4w..4w+3 (4 MFMA row tiles of 16 pixels) by the COB/16 channel
-// tiles for every tap, reading both operands from LDS.
+// reads with ds_read_b128 fall in distinct banks); wave w then multiplies the
+// COB/16 weight tiles by tile rows 4w..4w+3 (4 MFMA column tiles of 16
+// pixels) for every tap, reading both operands from LDS.
+//
+// The weights are the MFMA's A operand and the pixels its B operand: lane l
+// receives rows 4(l/16)..4(l/16)+3 of every weight tile's result for pixel
+// l % 16.  The weight rows are staged in the order that makes those rows the
+// COB/4 consecutive output channels (COB/4)(l/16)... (slot_channel), so each
+// lane holds whole 16-byte runs of channels of its pixels: the epilogue adds
+// the bias, applies the ReLU, rounds to BF16 and stores, pools (the pixel
+// pair x, x+1 sits in neighbouring lanes) or evaluates the fused head
+// straight from registers.
 //
 // The grid is persistent: a block walks the work items (tile, chunk) of its
 // output-channel block, tiles blockIdx.x, blockIdx.x + gridDim.x, ..., and
 // loads the next item's input and weights into registers while the MFMAs of
 // the current one run; the weights are loaded once when the input has a
-// single chunk.
+// single chunk.  The input pieces (8 channels of one halo pixel) are buffer
+// loads over the tile's image, at per-thread offsets fixed for the launch
+// plus one offset per tile; a piece outside the image (or the channels) is
+// given an offset past the image's end, which the buffer load returns as 0.
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
 constexpr int kLdsRow = 40;   // bf16 per staged pixel / weight row (32 + 8 pad)
 constexpr int kInQ = kHalo * kHalo * 4;            // 16-byte pieces of a staged input tile
 constexpr int kInPer = (kInQ + 255) / 256;         // per thread
+constexpr unsigned kOutside = 0x80000000u;         // a byte offset past every image
+
+// Output channel (within the block's COB) of result slot s = 16 j + m, m the
+// row of weight tile j's 16 x 16 result: lane group m / 4 holds channels
+// (COB/4)(m/4) ... (COB/4)(m/4) + COB/4 - 1, in the order 4 j + m % 4.
+template <int COB>
+__device__ __forceinline__ int slot_channel(int s) {
+  return (COB / 4) * ((s & 15) >> 2) + 4 * (s >> 4) + (s & 3);
+}
+
+// Buffer resource over `bytes` bytes at p: loads at larger offsets return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t image_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// two floats rounded to BF16 (nearest even, as f2bf) in one dword, lo first
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  union {
+    __bf16 h[2];
+    uint32_t u;
+  } r;
+  r.h[0] = (__bf16)lo;
+  r.h[1] = (__bf16)hi;
+  return r.u;
+}
+__device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
 template <typename TX, typename TY, int COB>
 __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY> a) {
   constexpr int CT = COB / 16;
+  constexpr int CL = COB / 4;                        // consecutive output channels per lane
   constexpr int kWPer = (9 * COB * 4 + 255) / 256;   // weight pieces per thread (9 taps)
+  constexpr int PW = sizeof(TX) / 2;                 // 16-byte loads per input piece
+  // COB 32 with BF16 input: the next chunk's weights are prefetched into
+  // registers with its input; otherwise (no registers to spare) they are
+  // loaded when stored, all pieces in flight at once
+  constexpr bool kWPrefetch = COB == 32 && PW == 1;
   __shared__ __align__(16) uint16_t s_in[kHalo * kHalo * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
-  // per wave: one row tile of BF16 output (16 pixels x COB channels), the
-  // transpose that turns the MFMA layout into 16-byte stores of 8 channels
-  __shared__ __align__(16) uint16_t s_out[4 * kTile * COB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
   const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
   const int ntiles = a.N * tiles_x * tiles_y;
-  // plain BF16 output (no mask, no accumulation), 8-channel aligned
-  const bool vec_out = sizeof(TY) == 2 && !a.mask && !a.accumulate && a.Cout % 8 == 0 && a.ldy % 8 == 0 &&
-                       a.cy0 % 8 == 0;
   const int nck = (a.Cin + 31) / 32;
   const int co0 = blockIdx.y * COB;
+  const int cl0 = co0 + CL * g;   // this lane's first output channel
   const int taps = a.taps;
   const int wq = taps * COB * 4;
   int tt = blockIdx.x;
   if (tt >= ntiles) return;
-  uint4 pin[kInPer];
+  float bl[CL];
+#pragma unroll
+  for (int k = 0; k < CL; k++) bl[k] = a.bias && cl0 + k < a.Cout ? a.bias[cl0 + k] : 0.f;
+  const int c8 = 8 * (tid & 3);   // channel group of every input piece of this thread
+  uint4 pin[kInPer][PW];
+  // the chunk's weights, kWPer 16-byte pieces per thread: slot row r = q / 4
+  // of tap r / COB holds output channel co0 + slot_channel(r % COB), input
+  // channels 8 (q % 4)... of the chunk; pieces outside the tensor read as 0
+  uint4 wreg[kWPer];
+  const __amdgpu_buffer_rsrc_t wrs = image_rsrc(a.w, (unsigned)((long)taps * a.Cout * a.Cin * 2));
+  auto wfetch = [&](int ck) {
+    int tq = tid;   // opaque: the offsets are not hoisted out of the tile loop
+    asm volatile("" : "+v"(tq));
+#pragma unroll
+    for (int k = 0; k < kWPer; k++) {
+      const int q = tq + 256 * k;
+      const int r = q >> 2, co = co0 + slot_channel<COB>(r % COB), t = r / COB, ci = 32 * ck + 8 * (q & 3);
+      const bool ok = q < wq && co < a.Cout && ci < a.Cin;
+      const unsigned off = ok ? (unsigned)(((t * a.Cout + co) * a.Cin + ci) * 2) : kOutside;
+      wreg[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(wrs, off, 0, 0));
+    }
+  };
+  auto wput = [&]() {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+#pragma unroll
+    for (int k = 0; k < kWPer; k++) {
+      const int q = tq + 256 * k;
+      if (q < wq) *reinterpret_cast<uint4*>(&s_w[(q >> 2) * kLdsRow + 8 * (q & 3)]) = wreg[k];
+    }
+  };
   // loads the input tile + halo of work item (tile, chunk) into registers
   auto fetch = [&](int tile, int ck) {
     const int tx = tile % tiles_x, rest = tile / tiles_x;
     const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
     const int ci0 = 32 * ck;
+    const bool cok = ci0 + c8 < a.Cin;
+    bool from_up = false;
+    if constexpr (sizeof(TX) == 2) from_up = ci0 < a.Cup;   // Cup is a multiple of 32
+    __amdgpu_buffer_rsrc_t rs;
+    int t0;
+    if (from_up) {
+      const int H2 = a.H >> 1, W2 = a.W >> 1;
+      rs = image_rsrc(a.up + (long)n * H2 * W2 * a.ldu, (unsigned)((long)H2 * W2 * a.ldu * 2));
+      t0 = ((y0 / 2 - 1) * W2 + (x0 / 2 - 1)) * a.ldu + a.cu0 + ci0;
+    } else {
+      rs = image_rsrc(a.x + (long)n * a.H * a.W * a.ldx, (unsigned)((long)a.H * a.W * a.ldx * sizeof(TX)));
+      t0 = ((y0 - 1) * a.W + (x0 - 1)) * a.ldx + a.cx0 + ci0;
+    }
+    // the pieces' geometry is recomputed per fetch from an opaque copy of the
+    // thread index: hoisted out of the tile loop it would be spilled
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
 #pragma unroll
     for (int i = 0; i < kInPer; i++) {
-      const int q = tid + 256 * i;
-      const int c = q & 3, pix = q >> 2;
-      const int yy = y0 + pix / kHalo - 1, xx = x0 + pix % kHalo - 1;
-      const int ci = ci0 + 8 * c;
-      const bool ok = q < kInQ && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ci < a.Cin;
-      V8 v;
-      const TX* src = a.x + (((long)n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.ldx + a.cx0 + (ok ? ci : 0);
-      if constexpr (sizeof(TX) == 2) {
-        // upsampled channels: the same load from the half-size tensor
-        if (ci < a.Cup)
-          src = a.up + (((long)n * (a.H >> 1) + (ok ? yy >> 1 : 0)) * (a.W >> 1) + (ok ? xx >> 1 : 0)) * a.ldu + a.cu0 +
-                (ok ? ci : 0);
+      // piece i: halo pixel (py, px), channels c8..c8+7 of the chunk
+      const int q = tq + 256 * i;
+      const int pix = q >> 2, py = pix / kHalo, px = pix - kHalo * py;
+      const bool ok = (kInQ % 1024 == 0 || q < kInQ) && cok && (unsigned)(y0 - 1 + py) < (unsigned)a.H &&
+                      (unsigned)(x0 - 1 + px) < (unsigned)a.W;
+      if (from_up) {
+        // (y0 - 1 + py) >> 1 = y0 / 2 - 1 + (py + 1) >> 1 for even y0
+        const int o = (int)__umul24(__umul24((py + 1) >> 1, a.W >> 1) + ((px + 1) >> 1), a.ldu) + c8;
+        const unsigned off = ok ? (unsigned)(t0 + o) * 2u : kOutside;
+        pin[i][0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+      } else {
+        const int o = (int)__umul24(__umul24(py, a.W) + px, a.ldx) + c8;
+        const unsigned off = ok ? (unsigned)(t0 + o) * (unsigned)sizeof(TX) : kOutside;
+#pragma unroll
+        for (int h = 0; h < PW; h++)
+          pin[i][h] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * h, 0, 0));
       }
-      v.v = load8<TX>(src, ok);
-      pin[i] = v.u;
     }
   };
   // stores the prefetched input and, when they change, the chunk's weights
   // (read straight from L2: every block of the launch reads the same ones)
   auto put = [&](int ck, bool with_w) {
+    int tq = tid;   // opaque: the addresses below are not hoisted out of the tile loop
+    asm volatile("" : "+v"(tq));
 #pragma unroll
     for (int i = 0; i < kInPer; i++) {
-      const int q = tid + 256 * i;
-      if (q < kInQ) *reinterpret_cast<uint4*>(&s_in[(q >> 2) * kLdsRow + 8 * (q & 3)]) = pin[i];
+      const int q = tq + 256 * i;
+      if (q < kInQ) {
+        uint4 v;
+        if constexpr (PW == 1) {
+          v = pin[i][0];
+        } else {
+          // FP32 input: 8 channels rounded to BF16
+          v.x = pack_bf16(__uint_as_float(pin[i][0].x), __uint_as_float(pin[i][0].y));
+          v.y = pack_bf16(__uint_as_float(pin[i][0].z), __uint_as_float(pin[i][0].w));
+          v.z = pack_bf16(__uint_as_float(pin[i][1].x), __uint_as_float(pin[i][1].y));
+          v.w = pack_bf16(__uint_as_float(pin[i][1].z), __uint_as_float(pin[i][1].w));
+        }
+        *reinterpret_cast<uint4*>(&s_in[(q >> 2) * kLdsRow + 8 * (q & 3)]) = v;
+      }
     }
     if (with_w) {
-      // in groups of 3: the group's loads are unconditional (piece 0 where
-      // the piece is padding, the value then dropped) and taken by an empty
-      // asm before any is stored, so that they are in flight together (a
-      // conditional load would be waited for before the next one is issued)
-      const int ci0 = 32 * ck;
-#pragma unroll
-      for (int i0 = 0; i0 < kWPer; i0 += 3) {
-        uint4 wv[3];
-        bool okv[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          if (i0 + k >= kWPer) break;
-          const int q = tid + 256 * (i0 + k);
-          const int c = q & 3, r = q >> 2;
-          const int co = co0 + r % COB, t = r / COB, ci = ci0 + 8 * c;
-          okv[k] = q < wq && co < a.Cout && ci < a.Cin;
-          wv[k] = *reinterpret_cast<const uint4*>(a.w + (okv[k] ? ((long)t * a.Cout + co) * a.Cin + ci : 0));
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          if (i0 + k >= kWPer) break;
-          asm volatile("" : "+v"(wv[k].x), "+v"(wv[k].y), "+v"(wv[k].z), "+v"(wv[k].w));
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          if (i0 + k >= kWPer) break;
-          const int q = tid + 256 * (i0 + k);
-          if (q < wq)
-            *reinterpret_cast<uint4*>(&s_w[(q >> 2) * kLdsRow + 8 * (q & 3)]) = okv[k] ? wv[k] : make_uint4(0, 0, 0, 0);
-        }
-      }
+      if constexpr (!kWPrefetch) wfetch(ck);
+      wput();
     }
   };
   f4 acc[4][CT];
@@ -206,6 +249,7 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
   int ck = 0;
   bool w_in = true;   // the weights in LDS are not this chunk's
   fetch(tt, 0);
+  if constexpr (kWPrefetch) wfetch(0);
   for (;;) {
     put(ck, w_in);
     __syncthreads();
@@ -217,164 +261,150 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
     }
     const bool more = tt2 < ntiles;
     w_in = nck > 1;
-    if (more) fetch(tt2, ck2);
+    if (more) {
+      fetch(tt2, ck2);
+      if constexpr (kWPrefetch)
+        if (w_in) wfetch(ck2);
+    }
     if (taps == 9) {
-      // for a column offset dx the A operand of tile row i at row offset dy is
-      // halo row 4w + i + dy: six halo rows serve all three dy
+      // for a column offset dx the B operand of tile row i at row offset dy
+      // is halo row 4w + i + dy: six halo rows serve all three dy
 #pragma unroll 1
       for (int dx = 0; dx < 3; dx++) {
-        bf8 A[6];
+        bf8 P[6];
 #pragma unroll
         for (int h = 0; h < 6; h++) {
           V8 v;
           v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + h) * kHalo + r16 + dx) * kLdsRow + 8 * g]);
-          A[h] = v.v;
+          P[h] = v.v;
         }
 #pragma unroll
         for (int dy = 0; dy < 3; dy++) {
           const int t = dy * 3 + dx;
-          bf8 B[CT];
+          bf8 Wt[CT];
 #pragma unroll
           for (int j = 0; j < CT; j++) {
             V8 v;
             v.u = *reinterpret_cast<const uint4*>(&s_w[(t * COB + 16 * j + r16) * kLdsRow + 8 * g]);
-            B[j] = v.v;
+            Wt[j] = v.v;
           }
 #pragma unroll
           for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < CT; j++)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i + dy], B[j], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wt[j], P[i + dy], acc[i][j], 0, 0, 0);
         }
       }
     } else {
       // 1x1: the centre of the halo
-      bf8 B[CT];
+      bf8 Wt[CT];
 #pragma unroll
       for (int j = 0; j < CT; j++) {
         V8 v;
         v.u = *reinterpret_cast<const uint4*>(&s_w[(16 * j + r16) * kLdsRow + 8 * g]);
-        B[j] = v.v;
+        Wt[j] = v.v;
       }
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         V8 v;
         v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + i + 1) * kHalo + r16 + 1) * kLdsRow + 8 * g]);
 #pragma unroll
-        for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v.v, B[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wt[j], v.v, acc[i][j], 0, 0, 0);
       }
     }
     if (ck == nck - 1) {
-      // epilogue: D[pixel 4g + r of row tile i][channel r16 of channel tile j]
+      // epilogue: acc[i][k / 4][k % 4] is channel cl0 + k of pixel
+      // (y0 + 4 wave + i, x0 + r16)
       const int tx = tt % tiles_x, rest = tt / tiles_x;
       const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
+      const int x = x0 + r16;
       if constexpr (sizeof(TY) == 2) {
-        if (vec_out) {
-          uint16_t* ep = s_out + wave * kTile * COB;
-          float bias[CT];
 #pragma unroll
-          for (int j = 0; j < CT; j++) {
-            const int co = co0 + 16 * j + r16;
-            bias[j] = a.bias && co < a.Cout ? a.bias[co] : 0.f;
-          }
-          // bias, ReLU and the rounding to BF16 once, in place: the stores
-          // and the pool below both read the rounded values
+        for (int i0 = 0; i0 < 4; i0 += 2) {
+          // rows i0, i0 + 1: bias, ReLU and the rounding to BF16 once (the
+          // stores, the pool and the head all use the rounded values)
+          uint32_t pk[2][CL / 2];
 #pragma unroll
-          for (int i = 0; i < 4; i++)
+          for (int h = 0; h < 2; h++) {
+            const int i = i0 + h;
 #pragma unroll
-            for (int j = 0; j < CT; j++)
-#pragma unroll
-              for (int r = 0; r < 4; r++) {
-                float v = acc[i][j][r] + bias[j];
-                if (a.relu) v = fmaxf(v, 0.f);
-                acc[i][j][r] = bf2f(f2bf(v));
+            for (int k = 0; k < CL; k += 2) {
+              float v0 = acc[i][k >> 2][k & 3] + bl[k];
+              float v1 = acc[i][(k + 1) >> 2][(k + 1) & 3] + bl[k + 1];
+              if (a.relu) {
+                v0 = fmaxf(v0, 0.f);
+                v1 = fmaxf(v1, 0.f);
               }
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const int y = y0 + 4 * wave + i;
-#pragma unroll
-            for (int j = 0; j < CT; j++)
-#pragma unroll
-              for (int r = 0; r < 4; r++)
-                ep[(4 * g + r) * COB + 16 * j + r16] = (uint16_t)(__float_as_uint(acc[i][j][r]) >> 16);
-            wave_sync();
-            if (a.head_out) {
-              // the head of the row tile's 16 pixels: lane = pixel + 16 x
-              // group of 8 channels (COB = Cout = 32); the group sums are
-              // added in group order, as k_head_fwd adds them
-              const int hp = lane & (kTile - 1), hg = lane >> 4;
-              V8 v;
-              v.u = *reinterpret_cast<const uint4*>(&ep[hp * COB + 8 * hg]);
-              const float part = head_group(v, a.head_w + 8 * hg, hg == 0 ? a.head_b[0] : 0.f);
-              const float p1 = __shfl(part, hp + 16), p2 = __shfl(part, hp + 32), p3 = __shfl(part, hp + 48);
-              const int x = x0 + hp;
-              if (hg == 0 && y < a.H && x < a.W) a.head_out[((long)n * a.H + y) * a.W + x] = ((part + p1) + p2) + p3;
-            } else {
-              // 16 pixels x COB/8 vectors of 8 channels
-#pragma unroll
-              for (int k = 0; k < COB / 32; k++) {
-                const int idx = lane + 64 * k;
-                const int px = idx / (COB / 8), c8 = idx % (COB / 8);
-                const int x = x0 + px, co = co0 + 8 * c8;
-                if (y < a.H && x < a.W && co < a.Cout)
-                  *reinterpret_cast<uint4*>(a.y + (((long)n * a.H + y) * a.W + x) * a.ldy + a.cy0 + co) =
-                      *reinterpret_cast<const uint4*>(&ep[px * COB + 8 * c8]);
-              }
+              pk[h][k / 2] = pack_bf16(v0, v1);
             }
-            wave_sync();
+            const int y = y0 + 4 * wave + i;
+            if (a.head_out) {
+              // lane group g holds the head's channel group g (COB = Cout
+              // = 32): its sum in channel order, then the groups' sums in
+              // group order, as k_head_fwd adds them
+              float s = g == 0 ? a.head_b[0] : 0.f;
+#pragma unroll
+              for (int k = 0; k < CL; k += 2) {
+                s = fmaf(bf_lo(pk[h][k / 2]), a.head_w[cl0 + k], s);
+                s = fmaf(bf_hi(pk[h][k / 2]), a.head_w[cl0 + k + 1], s);
+              }
+              const float s1 = __shfl(s, r16 + 16), s2 = __shfl(s, r16 + 32), s3 = __shfl(s, r16 + 48);
+              if (g == 0 && y < a.H && x < a.W) a.head_out[((long)n * a.H + y) * a.W + x] = ((s + s1) + s2) + s3;
+            } else if (y < a.H && x < a.W) {
+              uint16_t* dst = a.y + (((long)n * a.H + y) * a.W + x) * a.ldy + a.cy0 + cl0;
+#pragma unroll
+              for (int h8 = 0; h8 < CL / 8; h8++)
+                if (cl0 + 8 * h8 < a.Cout)
+                  *reinterpret_cast<uint4*>(dst + 8 * h8) =
+                      make_uint4(pk[h][4 * h8], pk[h][4 * h8 + 1], pk[h][4 * h8 + 2], pk[h][4 * h8 + 3]);
+            }
+          }
+          if (a.pool) {
+            // 2x2 average of the rounded values, summed in the order
+            // (y, x), (y, x+1), (y+1, x), (y+1, x+1); pixel x + 1 is the
+            // next lane's (DPP row_shl:1 reads lane l + 1 of the 16-lane row)
+            uint32_t nb[2][CL / 2];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+              for (int k = 0; k < CL / 2; k++)
+                nb[h][k] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pk[h][k], 0x101, 0xf, 0xf, true);
+            const int y = y0 + 4 * wave + i0;
+            if (!(r16 & 1) && y + 1 < a.H && x + 1 < a.W) {
+              uint32_t po[CL / 2];
+#pragma unroll
+              for (int k = 0; k < CL / 2; k++) {
+                const float lo = 0.25f * (((bf_lo(pk[0][k]) + bf_lo(nb[0][k])) + bf_lo(pk[1][k])) + bf_lo(nb[1][k]));
+                const float hi = 0.25f * (((bf_hi(pk[0][k]) + bf_hi(nb[0][k])) + bf_hi(pk[1][k])) + bf_hi(nb[1][k]));
+                po[k] = pack_bf16(lo, hi);
+              }
+              uint16_t* dst =
+                  a.pool + (((long)n * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.ldp + a.cp0 + cl0;
+#pragma unroll
+              for (int h8 = 0; h8 < CL / 8; h8++)
+                if (cl0 + 8 * h8 < a.Cout)
+                  *reinterpret_cast<uint4*>(dst + 8 * h8) =
+                      make_uint4(po[4 * h8], po[4 * h8 + 1], po[4 * h8 + 2], po[4 * h8 + 3]);
+            }
           }
         }
-      }
-#pragma unroll
-      for (int j = 0; j < CT; j++) {
-        const int co = co0 + 16 * j + r16;
-        if (co >= a.Cout) continue;
-        const float bias = a.bias ? a.bias[co] : 0.f;
+      } else {
+        // FP32 output (masked, accumulating), one element at a time
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int y = y0 + 4 * wave + i;
-          if (vec_out || y >= a.H) continue;
+          if (y >= a.H || x >= a.W) continue;
+          const long p = ((long)n * a.H + y) * a.W + x;
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int x = x0 + 4 * g + r;
-            if (x >= a.W) continue;
-            const long p = ((long)n * a.H + y) * a.W + x;
-            float v = acc[i][j][r] + bias;
+          for (int k = 0; k < CL; k++) {
+            const int co = cl0 + k;
+            if (co >= a.Cout) continue;
+            float v = acc[i][k >> 2][k & 3] + bl[k];
             if (a.relu) v = fmaxf(v, 0.f);
             if (a.mask && !(bf2f(a.mask[p * a.ldm + a.cm0 + co]) > 0.f)) v = 0.f;
             TY* dst = a.y + p * a.ldy + a.cy0 + co;
             if (a.accumulate) v += load1(dst);
             store1(dst, v);
-          }
-        }
-        if constexpr (sizeof(TY) == 2) {
-          if (a.pool) {
-            // 2x2 average of the stored BF16 values, summed in the order
-            // (y, x), (y, x+1), (y+1, x), (y+1, x+1): rows 4w+i and 4w+i+1,
-            // columns 4g+r and 4g+r+1 are all in this lane
-#pragma unroll
-            for (int i = 0; i < 4; i += 2) {
-              const int y = y0 + 4 * wave + i;
-              if (y + 1 >= a.H) continue;
-#pragma unroll
-              for (int r = 0; r < 4; r += 2) {
-                const int x = x0 + 4 * g + r;
-                if (x + 1 >= a.W) continue;
-                float q4[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                  if (vec_out) {
-                    q4[k] = acc[i + (k >> 1)][j][r + (k & 1)];   // already biased, ReLU'd and rounded
-                  } else {
-                    float v = acc[i + (k >> 1)][j][r + (k & 1)] + bias;
-                    if (a.relu) v = fmaxf(v, 0.f);
-                    q4[k] = bf2f(f2bf(v));
-                  }
-                }
-                const long pp = ((long)n * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
-                a.pool[pp * a.ldp + a.cp0 + co] = f2bf(0.25f * (q4[0] + q4[1] + q4[2] + q4[3]));
-              }
-            }
           }
         }
       }
@@ -747,6 +777,16 @@ hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
   if (a.head_out && (sizeof(TY) != 2 || a.Cout != 32 || a.mask || a.accumulate || a.pool || a.ldy % 8 ||
                      a.cy0 % 8 || !a.head_w || !a.head_b))
     return hipErrorInvalidValue;
+  // BF16 output is plain (no mask, no accumulation) and stored 8 channels at
+  // a time, as is its optional pool; FP32 output has no pool
+  if (sizeof(TY) == 2 && (a.mask || a.accumulate || a.Cout % 8 || a.ldy % 8 || a.cy0 % 8)) return hipErrorInvalidValue;
+  if (a.pool && (sizeof(TY) != 2 || a.ldp % 8 || a.cp0 % 8)) return hipErrorInvalidValue;
+  // input pieces are 8 channels; a 32-channel chunk comes from x or from up
+  // as a whole; one image's bytes are addressed by 32-bit buffer offsets
+  if (a.Cin % 8 || (a.up && (a.Cup % 32 || a.H % 2 || a.W % 2)) || (!a.up && a.Cup) ||
+      (long)a.H * a.W * a.ldx * (long)sizeof(TX) >= (1L << 31) || (long)a.H * a.W * a.ldu * 2 >= (1L << 31))
+    return hipErrorInvalidValue;
+  if (sizeof(TX) != 2 && a.up) return hipErrorInvalidValue;
   const long tiles = (long)a.N * ((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
   // persistent grid: the blocks that fit the device at once (LDS-bound:
   // 3 per CU at COB 32, 2 at COB 64), over all output-channel blocks
