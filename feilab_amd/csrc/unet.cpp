@@ -198,26 +198,31 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   // decoder convolutions read the upsampled half of their input straight
   // from the half-size tensor
   const bool fuse_up = !n->train;
-  R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, 96, 64, B, H, W, true, nullptr, 0, 0, false, false, st, nullptr,
+  // the skip halves: channels [2C, 3C) of cat in training, all of cat
+  // otherwise; a decoder reads its input channel ci >= Cup (the skip half)
+  // at channel ci - u of cat, u = Cup when only the skip half is stored
+  const int k1 = fuse_up ? 0 : 64, k2 = fuse_up ? 0 : 128, k3 = fuse_up ? 0 : 256;
+  const int u1 = fuse_up ? 64 : 0, u2 = fuse_up ? 128 : 0, u3 = fuse_up ? 256 : 0;
+  R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, n->cat1.C, k1, B, H, W, true, nullptr, 0, 0, false, false, st, nullptr,
          &n->p1));
   R(conv(n, cv[E2A], n->p1.p, 32, 0, n->a2.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
-  R(conv(n, cv[E2B], n->a2.p, 64, 0, n->cat2.p, 192, 128, B, H2, W2, true, nullptr, 0, 0, false, false, st, nullptr,
+  R(conv(n, cv[E2B], n->a2.p, 64, 0, n->cat2.p, n->cat2.C, k2, B, H2, W2, true, nullptr, 0, 0, false, false, st, nullptr,
          &n->p2));
   R(conv(n, cv[E3A], n->p2.p, 64, 0, n->a3.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st));
-  R(conv(n, cv[E3B], n->a3.p, 128, 0, n->cat3.p, 384, 256, B, H4, W4, true, nullptr, 0, 0, false, false, st, nullptr,
+  R(conv(n, cv[E3B], n->a3.p, 128, 0, n->cat3.p, n->cat3.C, k3, B, H4, W4, true, nullptr, 0, 0, false, false, st, nullptr,
          &n->p3));
   R(conv(n, cv[BA], n->p3.p, 128, 0, n->ab.p, 256, 0, B, H8, W8, true, nullptr, 0, 0, false, false, st));
   R(conv(n, cv[BB], n->ab.p, 256, 0, n->bb.p, 256, 0, B, H8, W8, true, nullptr, 0, 0, false, false, st));
   if (!fuse_up) R(up(n->bb, n->cat3, 0, B, H8, W8, st));
-  R(conv(n, cv[D3A], n->cat3.p, 384, 0, n->d3a.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st,
+  R(conv(n, cv[D3A], n->cat3.p, n->cat3.C, -u3, n->d3a.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st,
          fuse_up ? &n->bb : nullptr));
   R(conv(n, cv[D3B], n->d3a.p, 128, 0, n->d3b.p, 128, 0, B, H4, W4, true, nullptr, 0, 0, false, false, st));
   if (!fuse_up) R(up(n->d3b, n->cat2, 0, B, H4, W4, st));
-  R(conv(n, cv[D2A], n->cat2.p, 192, 0, n->d2a.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st,
+  R(conv(n, cv[D2A], n->cat2.p, n->cat2.C, -u2, n->d2a.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st,
          fuse_up ? &n->d3b : nullptr));
   R(conv(n, cv[D2B], n->d2a.p, 64, 0, n->d2b.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
   if (!fuse_up) R(up(n->d2b, n->cat1, 0, B, H2, W2, st));
-  R(conv(n, cv[D1A], n->cat1.p, 96, 0, n->d1a.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st,
+  R(conv(n, cv[D1A], n->cat1.p, n->cat1.C, -u1, n->d1a.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st,
          fuse_up ? &n->d2b : nullptr));
   if (n->train) {
     R(conv(n, cv[D1B], n->d1a.p, 32, 0, n->d1b.p, 32, 0, B, H, W, true, nullptr, 0, 0, false, false, st));
@@ -349,9 +354,13 @@ int icon_unet_create(icon_unet** out, int device, int max_batch, int H, int W, i
       (rc = alloc_bytes(n, (void**)&n->scale, sizeof(float) * max_batch)) ||
       (rc = alloc_bytes(n, (void**)&n->rms_part, sizeof(double) * max_batch * image_rms_parts())))
     return bail(rc);
-  if ((rc = t16(n, n->x8, P, 8)) || (rc = t16(n, n->a1, P, 32)) || (rc = t16(n, n->cat1, P, 96)) ||
-      (rc = t16(n, n->p1, P2, 32)) || (rc = t16(n, n->a2, P2, 64)) || (rc = t16(n, n->cat2, P2, 192)) ||
-      (rc = t16(n, n->p2, P4, 64)) || (rc = t16(n, n->a3, P4, 128)) || (rc = t16(n, n->cat3, P4, 384)) ||
+  // cat1..3: the decoders' concatenated inputs [upsampled | skip] for
+  // training; without training the upsampled half is read from the
+  // half-size tensor and only the skip half is stored, densely
+  const int sk = n->train ? 3 : 1;
+  if ((rc = t16(n, n->x8, P, 8)) || (rc = t16(n, n->a1, P, 32)) || (rc = t16(n, n->cat1, P, 32 * sk)) ||
+      (rc = t16(n, n->p1, P2, 32)) || (rc = t16(n, n->a2, P2, 64)) || (rc = t16(n, n->cat2, P2, 64 * sk)) ||
+      (rc = t16(n, n->p2, P4, 64)) || (rc = t16(n, n->a3, P4, 128)) || (rc = t16(n, n->cat3, P4, 128 * sk)) ||
       (rc = t16(n, n->p3, P8, 128)) || (rc = t16(n, n->ab, P8, 256)) || (rc = t16(n, n->bb, P8, 256)) ||
       (rc = t16(n, n->d3a, P4, 128)) || (rc = t16(n, n->d3b, P4, 128)) || (rc = t16(n, n->d2a, P2, 64)) ||
       (rc = t16(n, n->d2b, P2, 64)) || (rc = t16(n, n->d1a, P, 32)) || (rc = t16(n, n->d1b, P, 32)))

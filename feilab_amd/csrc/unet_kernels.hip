@@ -86,8 +86,6 @@ __device__ __forceinline__ void store1(float* p, float v) { *p = v; }
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
 constexpr int kLdsRow = 40;   // bf16 per staged pixel / weight row (32 + 8 pad)
-constexpr int kInQ = kHalo * kHalo * 4;            // 16-byte pieces of a staged input tile
-constexpr int kInPer = (kInQ + 255) / 256;         // per thread
 constexpr unsigned kOutside = 0x80000000u;         // a byte offset past every image
 
 // Output channel (within the block's COB) of result slot s = 16 j + m, m the
@@ -103,6 +101,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t image_rsrc(const void* p, unsi
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
 
+// s_waitcnt vmcnt(0).  A load whose value is stored under a per-lane
+// condition is waited for only on the branch that stores it; without this
+// wait on every path the compiler makes the next writes of the load's
+// registers (the MFMA operands' LDS reads) wait for all loads in flight,
+// the next item's prefetch included.
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // two floats rounded to BF16 (nearest even, as f2bf) in one dword, lo first
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
   union {
@@ -116,134 +121,186 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
 __device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
-template <typename TX, typename TY, int COB>
-__global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY> a) {
+template <typename TX, typename TY, int COB, int R>
+__global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvArgs<TX, TY> a) {
   constexpr int CT = COB / 16;
   constexpr int CL = COB / 4;                        // consecutive output channels per lane
-  constexpr int kWPer = (9 * COB * 4 + 255) / 256;   // weight pieces per thread (9 taps)
   constexpr int PW = sizeof(TX) / 2;                 // 16-byte loads per input piece
   // COB 32 with BF16 input: the next chunk's weights are prefetched into
   // registers with its input; otherwise (no registers to spare) they are
   // loaded when stored, all pieces in flight at once
   constexpr bool kWPrefetch = COB == 32 && PW == 1;
-  __shared__ __align__(16) uint16_t s_in[kHalo * kHalo * kLdsRow];
+  constexpr int kTH = 4 * R;       // tile rows: R per wave
+  constexpr int kHY = kTH + 2;     // halo rows
+  __shared__ __align__(16) uint16_t s_in[kHY * kHalo * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
-  const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
+  const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTH - 1) / kTH;
   const int ntiles = a.N * tiles_x * tiles_y;
   const int nck = (a.Cin + 31) / 32;
   const int co0 = blockIdx.y * COB;
   const int cl0 = co0 + CL * g;   // this lane's first output channel
   const int taps = a.taps;
-  const int wq = taps * COB * 4;
   int tt = blockIdx.x;
   if (tt >= ntiles) return;
   float bl[CL];
 #pragma unroll
   for (int k = 0; k < CL; k++) bl[k] = a.bias && cl0 + k < a.Cout ? a.bias[cl0 + k] : 0.f;
-  const int c8 = 8 * (tid & 3);   // channel group of every input piece of this thread
-  uint4 pin[kInPer][PW];
-  // the chunk's weights, kWPer 16-byte pieces per thread: slot row r = q / 4
-  // of tap r / COB holds output channel co0 + slot_channel(r % COB), input
-  // channels 8 (q % 4)... of the chunk; pieces outside the tensor read as 0
-  uint4 wreg[kWPer];
-  const __amdgpu_buffer_rsrc_t wrs = image_rsrc(a.w, (unsigned)((long)taps * a.Cout * a.Cin * 2));
-  auto wfetch = [&](int ck) {
-    int tq = tid;   // opaque: the offsets are not hoisted out of the tile loop
-    asm volatile("" : "+v"(tq));
+  // the fused head's weights of this lane's channels (COB 32 only), loaded
+  // here: a load in the epilogue would wait for the prefetch in flight
+  constexpr int HW = COB == 32 ? CL : 1;
+  float hw[HW], hb = 0.f;
+  if constexpr (COB == 32) {
+    if (a.head_out) {
 #pragma unroll
-    for (int k = 0; k < kWPer; k++) {
-      const int q = tq + 256 * k;
-      const int r = q >> 2, co = co0 + slot_channel<COB>(r % COB), t = r / COB, ci = 32 * ck + 8 * (q & 3);
-      const bool ok = q < wq && co < a.Cout && ci < a.Cin;
-      const unsigned off = ok ? (unsigned)(((t * a.Cout + co) * a.Cin + ci) * 2) : kOutside;
-      wreg[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(wrs, off, 0, 0));
+      for (int k = 0; k < HW; k++) hw[k] = a.head_w[cl0 + k];
+      hb = g == 0 ? a.head_b[0] : 0.f;
+    }
+  }
+  // ---- staging.  One wave instruction moves 16 consecutive pixels (or
+  // weight rows) x 4 pieces of 8 channels: lane l takes pixel / row l % 16
+  // and piece (l / 16) % 4, so the 16 lanes of one piece write 16
+  // consecutive 80-byte LDS rows (distinct banks) and the wave reads 1 KB of
+  // consecutive memory.  Offsets that do not change during the launch are
+  // computed here once; a piece outside the image or the tensor is given
+  // the offset kOutside, which the buffer load returns as 0.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  const int lx = lane & 15, lc8 = 8 * ((lane >> 4) & 3);
+  // Input, main part: halo rows 0..kHY-1 x halo columns 0..15; wave w takes
+  // the rows w, w + 4, ... (kRounds rounds, the last one waves 0 and 1 only).
+  // Byte offset of this lane's piece from halo column 0 of a row, in x and
+  // in the half-size tensor up:
+  constexpr int kRounds = (kHY + 3) / 4;
+  const unsigned lo_x = (unsigned)((lx * a.ldx + lc8) * (int)sizeof(TX));
+  const unsigned lo_u = (unsigned)((((lx + 1) >> 1) * a.ldu + lc8) * 2);
+  // Input, extra part: halo columns 16 and 17 of every row, 8 kHY pieces in
+  // kExR rounds; piece e = t + 256 round: row e / 8, column 16 + (e / 4) % 2,
+  // piece e % 4.
+  constexpr int kExR = (8 * kHY + 255) / 256;
+  const int ex_x = kTile + ((tid >> 2) & 1), ex_c8 = 8 * (tid & 3);
+  // LDS element offsets of this lane's pieces (main: round 0; round r is
+  // 4 r halo rows further)
+  const int la_in = (wave * kHalo + lx) * kLdsRow + lc8;
+  const int la_ex = ((tid >> 3) * kHalo + ex_x) * kLdsRow + ex_c8;   // round e: + 32 e halo rows
+  // Weights: slot row r = tap * COB + slot; round k covers rows 64 k ... 64 k
+  // + 63, wave w the rows 64 k + 16 w + (0..15), so the tap (64 k + 16 w) /
+  // COB of a round is uniform and the lane's slot (16 w + lx) % COB fixed.
+  constexpr int kWRounds = (9 * COB + 63) / 64;
+  const int wco = co0 + slot_channel<COB>((16 * wave + lx) % COB);
+  const unsigned lo_w = wco < a.Cout ? (unsigned)((wco * a.Cin + lc8) * 2) : kOutside;
+  const int la_w = (16 * wave + lx) * kLdsRow + lc8;
+  uint4 pin[kRounds + kExR][PW];
+  uint4 wreg[kWRounds];
+  // loads the chunk's weights into registers
+  const uint16_t* const Wt0 = a.w;
+  const long wtap = (long)a.Cout * a.Cin;   // elements per tap
+  auto wfetch = [&](int ck) {
+    const uint16_t* wck = Wt0 + 32 * ck;
+#pragma unroll
+    for (int k = 0; k < kWRounds; k++) {
+      const int t = (64 * k + 16 * wv) / COB;
+      wreg[k] = make_uint4(0, 0, 0, 0);
+      if (t < taps) {
+        const auto rs = image_rsrc(wck + t * wtap, 0x7fffffffu);
+        wreg[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, lo_w, 0, 0));
+      }
     }
   };
   auto wput = [&]() {
-    int tq = tid;
-    asm volatile("" : "+v"(tq));
 #pragma unroll
-    for (int k = 0; k < kWPer; k++) {
-      const int q = tq + 256 * k;
-      if (q < wq) *reinterpret_cast<uint4*>(&s_w[(q >> 2) * kLdsRow + 8 * (q & 3)]) = wreg[k];
-    }
+    for (int k = 0; k < kWRounds; k++)
+      if ((64 * k + 16 * wv) / COB < taps) *reinterpret_cast<uint4*>(&s_w[la_w + 64 * k * kLdsRow]) = wreg[k];
+    vm_wait_all();
   };
   // loads the input tile + halo of work item (tile, chunk) into registers
+  const TX* const X = a.x;
+  const uint16_t* const UP = a.up;
+  const int H = a.H, W = a.W, H2 = a.H >> 1, W2 = a.W >> 1, ldx = a.ldx, ldu = a.ldu;
   auto fetch = [&](int tile, int ck) {
     const int tx = tile % tiles_x, rest = tile / tiles_x;
-    const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
+    const int y0 = (rest % tiles_y) * kTH, x0 = tx * kTile, n = rest / tiles_y;
     const int ci0 = 32 * ck;
-    const bool cok = ci0 + c8 < a.Cin;
     bool from_up = false;
     if constexpr (sizeof(TX) == 2) from_up = ci0 < a.Cup;   // Cup is a multiple of 32
-    __amdgpu_buffer_rsrc_t rs;
-    int t0;
+    // halo row wv (image row y0 - 1 + wv) from halo column 0 (image column
+    // x0 - 1; its offset is not used where that column is outside the
+    // image), and the step of 4 halo rows (2 rows of up), in bytes
+    const int yw = y0 - 1 + wv;
+    const char* row0;
+    long step;
     if (from_up) {
-      const int H2 = a.H >> 1, W2 = a.W >> 1;
-      rs = image_rsrc(a.up + (long)n * H2 * W2 * a.ldu, (unsigned)((long)H2 * W2 * a.ldu * 2));
-      t0 = ((y0 / 2 - 1) * W2 + (x0 / 2 - 1)) * a.ldu + a.cu0 + ci0;
+      row0 = reinterpret_cast<const char*>(UP + (((long)n * H2 + (yw >> 1)) * W2 + (x0 >> 1) - 1) * ldu + a.cu0 + ci0);
+      step = 2L * W2 * ldu * 2;
     } else {
-      rs = image_rsrc(a.x + (long)n * a.H * a.W * a.ldx, (unsigned)((long)a.H * a.W * a.ldx * sizeof(TX)));
-      t0 = ((y0 - 1) * a.W + (x0 - 1)) * a.ldx + a.cx0 + ci0;
+      row0 = reinterpret_cast<const char*>(X + (((long)n * H + yw) * W + x0 - 1) * ldx + a.cx0 + ci0);
+      step = 4L * W * ldx * (long)sizeof(TX);
     }
-    // the pieces' geometry is recomputed per fetch from an opaque copy of the
-    // thread index: hoisted out of the tile loop it would be spilled
-    int tq = tid;
-    asm volatile("" : "+v"(tq));
+    const unsigned lo = (unsigned)(x0 - 1 + lx) < (unsigned)W ? (from_up ? lo_u : lo_x) : kOutside;
 #pragma unroll
-    for (int i = 0; i < kInPer; i++) {
-      // piece i: halo pixel (py, px), channels c8..c8+7 of the chunk
-      const int q = tq + 256 * i;
-      const int pix = q >> 2, py = pix / kHalo, px = pix - kHalo * py;
-      const bool ok = (kInQ % 1024 == 0 || q < kInQ) && cok && (unsigned)(y0 - 1 + py) < (unsigned)a.H &&
-                      (unsigned)(x0 - 1 + px) < (unsigned)a.W;
-      if (from_up) {
-        // (y0 - 1 + py) >> 1 = y0 / 2 - 1 + (py + 1) >> 1 for even y0
-        const int o = (int)__umul24(__umul24((py + 1) >> 1, a.W >> 1) + ((px + 1) >> 1), a.ldu) + c8;
-        const unsigned off = ok ? (unsigned)(t0 + o) * 2u : kOutside;
-        pin[i][0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
-      } else {
-        const int o = (int)__umul24(__umul24(py, a.W) + px, a.ldx) + c8;
-        const unsigned off = ok ? (unsigned)(t0 + o) * (unsigned)sizeof(TX) : kOutside;
+    for (int r = 0; r < kRounds; r++) {
+#pragma unroll
+      for (int h = 0; h < PW; h++) pin[r][h] = make_uint4(0, 0, 0, 0);
+      if ((r < kRounds - 1 || wv + 4 * r < kHY) && (unsigned)(yw + 4 * r) < (unsigned)H) {
+        const auto rs = image_rsrc(row0 + r * step, 0x7fffffffu);
 #pragma unroll
         for (int h = 0; h < PW; h++)
-          pin[i][h] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * h, 0, 0));
+          pin[r][h] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, lo + 16 * h, 0, 0));
       }
+    }
+#pragma unroll
+    for (int e = 0; e < kExR; e++) {
+      const int yy = y0 - 1 + (tid >> 3) + 32 * e, xx = x0 - 1 + ex_x;
+      const bool ok = tid + 256 * e < 8 * kHY && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
+      const char* img;
+      unsigned off;
+      if (from_up) {
+        img = reinterpret_cast<const char*>(UP + (long)n * H2 * W2 * ldu + a.cu0 + ci0);
+        off = (unsigned)((((yy >> 1) * W2 + (xx >> 1)) * ldu + ex_c8) * 2);
+      } else {
+        img = reinterpret_cast<const char*>(X + (long)n * H * W * ldx + a.cx0 + ci0);
+        off = (unsigned)(((yy * W + xx) * ldx + ex_c8) * (int)sizeof(TX));
+      }
+      const auto rs = image_rsrc(img, 0x7fffffffu);
+#pragma unroll
+      for (int h = 0; h < PW; h++)
+        pin[kRounds + e][h] =
+            __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? off + 16 * h : kOutside, 0, 0));
     }
   };
   // stores the prefetched input and, when they change, the chunk's weights
   // (read straight from L2: every block of the launch reads the same ones)
   auto put = [&](int ck, bool with_w) {
-    int tq = tid;   // opaque: the addresses below are not hoisted out of the tile loop
-    asm volatile("" : "+v"(tq));
-#pragma unroll
-    for (int i = 0; i < kInPer; i++) {
-      const int q = tq + 256 * i;
-      if (q < kInQ) {
+    auto bf = [&](const uint4* p) {
+      if constexpr (PW == 1) {
+        return p[0];
+      } else {
+        // FP32 input: 8 channels rounded to BF16
         uint4 v;
-        if constexpr (PW == 1) {
-          v = pin[i][0];
-        } else {
-          // FP32 input: 8 channels rounded to BF16
-          v.x = pack_bf16(__uint_as_float(pin[i][0].x), __uint_as_float(pin[i][0].y));
-          v.y = pack_bf16(__uint_as_float(pin[i][0].z), __uint_as_float(pin[i][0].w));
-          v.z = pack_bf16(__uint_as_float(pin[i][1].x), __uint_as_float(pin[i][1].y));
-          v.w = pack_bf16(__uint_as_float(pin[i][1].z), __uint_as_float(pin[i][1].w));
-        }
-        *reinterpret_cast<uint4*>(&s_in[(q >> 2) * kLdsRow + 8 * (q & 3)]) = v;
+        v.x = pack_bf16(__uint_as_float(p[0].x), __uint_as_float(p[0].y));
+        v.y = pack_bf16(__uint_as_float(p[0].z), __uint_as_float(p[0].w));
+        v.z = pack_bf16(__uint_as_float(p[1].x), __uint_as_float(p[1].y));
+        v.w = pack_bf16(__uint_as_float(p[1].z), __uint_as_float(p[1].w));
+        return v;
       }
-    }
+    };
+#pragma unroll
+    for (int r = 0; r < kRounds; r++)
+      if (r < kRounds - 1 || wv + 4 * r < kHY)
+        *reinterpret_cast<uint4*>(&s_in[la_in + 4 * r * kHalo * kLdsRow]) = bf(pin[r]);
+#pragma unroll
+    for (int e = 0; e < kExR; e++)
+      if (tid + 256 * e < 8 * kHY) *reinterpret_cast<uint4*>(&s_in[la_ex + 32 * e * kHalo * kLdsRow]) = bf(pin[kRounds + e]);
+    vm_wait_all();
     if (with_w) {
       if constexpr (!kWPrefetch) wfetch(ck);
       wput();
     }
   };
-  f4 acc[4][CT];
+  f4 acc[R][CT];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < R; i++)
 #pragma unroll
     for (int j = 0; j < CT; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   int ck = 0;
@@ -268,14 +325,14 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
     }
     if (taps == 9) {
       // for a column offset dx the B operand of tile row i at row offset dy
-      // is halo row 4w + i + dy: six halo rows serve all three dy
+      // is halo row R w + i + dy: R + 2 halo rows serve all three dy
 #pragma unroll 1
       for (int dx = 0; dx < 3; dx++) {
-        bf8 P[6];
+        bf8 P[R + 2];
 #pragma unroll
-        for (int h = 0; h < 6; h++) {
+        for (int h = 0; h < R + 2; h++) {
           V8 v;
-          v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + h) * kHalo + r16 + dx) * kLdsRow + 8 * g]);
+          v.u = *reinterpret_cast<const uint4*>(&s_in[((R * wave + h) * kHalo + r16 + dx) * kLdsRow + 8 * g]);
           P[h] = v.v;
         }
 #pragma unroll
@@ -289,7 +346,7 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
             Wt[j] = v.v;
           }
 #pragma unroll
-          for (int i = 0; i < 4; i++)
+          for (int i = 0; i < R; i++)
 #pragma unroll
             for (int j = 0; j < CT; j++)
               acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wt[j], P[i + dy], acc[i][j], 0, 0, 0);
@@ -305,22 +362,22 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
         Wt[j] = v.v;
       }
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < R; i++) {
         V8 v;
-        v.u = *reinterpret_cast<const uint4*>(&s_in[((4 * wave + i + 1) * kHalo + r16 + 1) * kLdsRow + 8 * g]);
+        v.u = *reinterpret_cast<const uint4*>(&s_in[((R * wave + i + 1) * kHalo + r16 + 1) * kLdsRow + 8 * g]);
 #pragma unroll
         for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wt[j], v.v, acc[i][j], 0, 0, 0);
       }
     }
     if (ck == nck - 1) {
       // epilogue: acc[i][k / 4][k % 4] is channel cl0 + k of pixel
-      // (y0 + 4 wave + i, x0 + r16)
+      // (y0 + R wave + i, x0 + r16)
       const int tx = tt % tiles_x, rest = tt / tiles_x;
-      const int y0 = (rest % tiles_y) * kTile, x0 = tx * kTile, n = rest / tiles_y;
+      const int y0 = (rest % tiles_y) * kTH, x0 = tx * kTile, n = rest / tiles_y;
       const int x = x0 + r16;
       if constexpr (sizeof(TY) == 2) {
 #pragma unroll
-        for (int i0 = 0; i0 < 4; i0 += 2) {
+        for (int i0 = 0; i0 < R; i0 += 2) {
           // rows i0, i0 + 1: bias, ReLU and the rounding to BF16 once (the
           // stores, the pool and the head all use the rounded values)
           uint32_t pk[2][CL / 2];
@@ -337,16 +394,16 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
               }
               pk[h][k / 2] = pack_bf16(v0, v1);
             }
-            const int y = y0 + 4 * wave + i;
-            if (a.head_out) {
+            const int y = y0 + R * wave + i;
+            if (COB == 32 && a.head_out) {
               // lane group g holds the head's channel group g (COB = Cout
               // = 32): its sum in channel order, then the groups' sums in
               // group order, as k_head_fwd adds them
-              float s = g == 0 ? a.head_b[0] : 0.f;
+              float s = hb;
 #pragma unroll
-              for (int k = 0; k < CL; k += 2) {
-                s = fmaf(bf_lo(pk[h][k / 2]), a.head_w[cl0 + k], s);
-                s = fmaf(bf_hi(pk[h][k / 2]), a.head_w[cl0 + k + 1], s);
+              for (int k = 0; k < HW; k += 2) {
+                s = fmaf(bf_lo(pk[h][k / 2]), hw[k], s);
+                s = fmaf(bf_hi(pk[h][k / 2]), hw[k + 1], s);
               }
               const float s1 = __shfl(s, r16 + 16), s2 = __shfl(s, r16 + 32), s3 = __shfl(s, r16 + 48);
               if (g == 0 && y < a.H && x < a.W) a.head_out[((long)n * a.H + y) * a.W + x] = ((s + s1) + s2) + s3;
@@ -369,7 +426,7 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
 #pragma unroll
               for (int k = 0; k < CL / 2; k++)
                 nb[h][k] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pk[h][k], 0x101, 0xf, 0xf, true);
-            const int y = y0 + 4 * wave + i0;
+            const int y = y0 + R * wave + i0;
             if (!(r16 & 1) && y + 1 < a.H && x + 1 < a.W) {
               uint32_t po[CL / 2];
 #pragma unroll
@@ -391,8 +448,8 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
       } else {
         // FP32 output (masked, accumulating), one element at a time
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int y = y0 + 4 * wave + i;
+        for (int i = 0; i < R; i++) {
+          const int y = y0 + R * wave + i;
           if (y >= a.H || x >= a.W) continue;
           const long p = ((long)n * a.H + y) * a.W + x;
 #pragma unroll
@@ -409,7 +466,7 @@ __global__ __launch_bounds__(256, COB == 32 ? 3 : 2) void k_conv(ConvArgs<TX, TY
         }
       }
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+      for (int i = 0; i < R; i++)
 #pragma unroll
         for (int j = 0; j < CT; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     }
@@ -770,6 +827,21 @@ __global__ void k_zero(float* p, long n) {
 
 static int blocks_for(long n, int per = 256) { return (int)std::min<long>((n + per - 1) / per, 65536); }
 
+// Rows per wave of the 32-channel convolutions with BF16 input (tiles of
+// 4 kRows32 x 16 pixels); the others use 4 (their registers are full).
+constexpr int kRows32 = 8;
+
+template <typename TX, typename TY, int COB, int R>
+void launch_conv_tiles(const ConvArgs<TX, TY>& a, int cus, hipStream_t st) {
+  // LDS per block: the staged halo tile and one chunk's weights
+  constexpr long lds = ((4L * R + 2) * kHalo + 9L * COB) * kLdsRow * 2;
+  constexpr long per_cu = std::min<long>(COB == 32 && R == 4 ? 3 : 2, (160L * 1024) / lds);
+  const long tiles = (long)a.N * ((a.H + 4 * R - 1) / (4 * R)) * ((a.W + kTile - 1) / kTile);
+  const int cb = (a.Cout + COB - 1) / COB;
+  const long gx = std::max<long>(1, std::min<long>(tiles, per_cu * cus / cb));
+  hipLaunchKernelGGL((k_conv<TX, TY, COB, R>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
+}
+
 template <typename TX, typename TY>
 hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
   if (a.taps != 9 && a.taps != 1) return hipErrorInvalidValue;
@@ -781,30 +853,24 @@ hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
   // a time, as is its optional pool; FP32 output has no pool
   if (sizeof(TY) == 2 && (a.mask || a.accumulate || a.Cout % 8 || a.ldy % 8 || a.cy0 % 8)) return hipErrorInvalidValue;
   if (a.pool && (sizeof(TY) != 2 || a.ldp % 8 || a.cp0 % 8)) return hipErrorInvalidValue;
-  // input pieces are 8 channels; a 32-channel chunk comes from x or from up
-  // as a whole; one image's bytes are addressed by 32-bit buffer offsets
-  if (a.Cin % 8 || (a.up && (a.Cup % 32 || a.H % 2 || a.W % 2)) || (!a.up && a.Cup) ||
+  // the input is staged in whole 32-channel chunks, each from x or from up;
+  // one image's bytes are addressed by 32-bit buffer offsets
+  if (a.Cin % 32 || (a.up && (a.Cup % 32 || a.H % 2 || a.W % 2)) || (!a.up && a.Cup) ||
       (long)a.H * a.W * a.ldx * (long)sizeof(TX) >= (1L << 31) || (long)a.H * a.W * a.ldu * 2 >= (1L << 31))
     return hipErrorInvalidValue;
   if (sizeof(TX) != 2 && a.up) return hipErrorInvalidValue;
-  const long tiles = (long)a.N * ((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
-  // persistent grid: the blocks that fit the device at once (LDS-bound:
-  // 3 per CU at COB 32, 2 at COB 64), over all output-channel blocks
+  // persistent grid: the blocks that fit the device at once (LDS-bound), over
+  // all output-channel blocks
   static int cus = 0;
   if (!cus) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 256;
   }
-  if (a.Cout <= 32) {
-    const int cb = (a.Cout + 31) / 32;
-    const long gx = std::max<long>(1, std::min<long>(tiles, 3L * cus / cb));
-    hipLaunchKernelGGL((k_conv<TX, TY, 32>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
-  } else {
-    const int cb = (a.Cout + 63) / 64;
-    const long gx = std::max<long>(1, std::min<long>(tiles, 2L * cus / cb));
-    hipLaunchKernelGGL((k_conv<TX, TY, 64>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
-  }
+  if (a.Cout <= 32)
+    launch_conv_tiles<TX, TY, 32, sizeof(TX) == 2 ? kRows32 : 4>(a, cus, st);
+  else
+    launch_conv_tiles<TX, TY, 64, 4>(a, cus, st);
   return hipGetLastError();
 }
 template hipError_t launch_conv<uint16_t, uint16_t>(const ConvArgs<uint16_t, uint16_t>&, hipStream_t);
