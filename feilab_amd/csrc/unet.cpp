@@ -136,6 +136,15 @@ int refresh_bf16(icon_unet* n, hipStream_t st) {
   return 0;
 }
 
+// The network's first convolution computed inside the second one's input
+// staging (from the FP32 images, divided by scale when given); its output
+// written to a1_out when given.
+struct FirstConv {
+  const float* img;
+  const float* scale;
+  uint16_t* a1_out;
+};
+
 // up (optional): the input's first Cup channels, read from this half-size
 // tensor (nearest-neighbour 2x upsampling) instead of x; pool (optional):
 // the output's 2x2 average, written at half size
@@ -143,7 +152,7 @@ template <typename TX, typename TY>
 int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int ldy, int cy0, int B, int H, int W,
          bool relu, const uint16_t* mask, int ldm, int cm0, bool flipped, bool accumulate, hipStream_t st,
          const T16* up = nullptr, const T16* pool = nullptr, const float* head_w = nullptr,
-         const float* head_b = nullptr, float* head_out = nullptr) {
+         const float* head_b = nullptr, float* head_out = nullptr, const FirstConv* first = nullptr) {
   ConvArgs<TX, TY> a{};
   a.x = x; a.ldx = ldx; a.cx0 = cx0;
   a.w = flipped ? c.wt16 : c.w16;
@@ -158,6 +167,13 @@ int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int 
   if (up) { a.up = up->p; a.ldu = up->C; a.cu0 = 0; a.Cup = up->C; }
   if (pool) { a.pool = pool->p; a.ldp = pool->C; a.cp0 = 0; }
   a.head_w = head_w; a.head_b = head_b; a.head_out = head_out;
+  if constexpr (sizeof(TX) == 2 && sizeof(TY) == 2) {
+    if (first) {
+      const Conv& c1 = n->convs[0];
+      a.x1 = first->img; a.x1_scale = first->scale; a.a1_out = first->a1_out;
+      a.w1 = c1.w16; a.w1_cin = c1.cin; a.b1 = n->params + c1.off_b;
+    }
+  }
   UCHK(launch_conv(a, st));
   return 0;
 }
@@ -188,11 +204,15 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   auto& cv = n->convs;
   int rc;
 #define R(x) do { if ((rc = (x))) return rc; } while (0)
-  const hipError_t ce = in_scale ? launch_to_bf16_scaled(d_in, in_scale, (long)H * W, B, n->x8.p, st)
-                                 : launch_f32_to_bf16(d_in, n->x8.p, P, 8, 0, 1, st);
-  if (ce != hipSuccess) return ufail(2, "input conversion: %s", hipGetErrorString(ce));
-  UCHK(launch_conv_first(n->x8.p, 8, cv[E1A].w16, cv[E1A].cin, n->params + cv[E1A].off_b, n->a1.p, 32, B, H, W, 32,
-                         st));
+  if (n->train) {
+    // the BF16 input for the first convolution's weight gradient
+    const hipError_t ce = in_scale ? launch_to_bf16_scaled(d_in, in_scale, (long)H * W, B, n->x8.p, st)
+                                   : launch_f32_to_bf16(d_in, n->x8.p, P, 8, 0, 1, st);
+    if (ce != hipSuccess) return ufail(2, "input conversion: %s", hipGetErrorString(ce));
+  }
+  // the first convolution (E1A) inside E1B's input staging; its output is
+  // stored for the backward pass only
+  const FirstConv first{d_in, in_scale, n->train ? n->a1.p : nullptr};
   // the encoder convolutions also write their 2x2 average pool; without
   // training (no weight gradients need the concatenated decoder inputs) the
   // decoder convolutions read the upsampled half of their input straight
@@ -204,7 +224,7 @@ int forward(icon_unet* n, int B, const float* d_in, float* d_out, hipStream_t st
   const int k1 = fuse_up ? 0 : 64, k2 = fuse_up ? 0 : 128, k3 = fuse_up ? 0 : 256;
   const int u1 = fuse_up ? 64 : 0, u2 = fuse_up ? 128 : 0, u3 = fuse_up ? 256 : 0;
   R(conv(n, cv[E1B], n->a1.p, 32, 0, n->cat1.p, n->cat1.C, k1, B, H, W, true, nullptr, 0, 0, false, false, st, nullptr,
-         &n->p1));
+         &n->p1, nullptr, nullptr, nullptr, &first));
   R(conv(n, cv[E2A], n->p1.p, 32, 0, n->a2.p, 64, 0, B, H2, W2, true, nullptr, 0, 0, false, false, st));
   R(conv(n, cv[E2B], n->a2.p, 64, 0, n->cat2.p, n->cat2.C, k2, B, H2, W2, true, nullptr, 0, 0, false, false, st, nullptr,
          &n->p2));
@@ -358,7 +378,9 @@ int icon_unet_create(icon_unet** out, int device, int max_batch, int H, int W, i
   // training; without training the upsampled half is read from the
   // half-size tensor and only the skip half is stored, densely
   const int sk = n->train ? 3 : 1;
-  if ((rc = t16(n, n->x8, P, 8)) || (rc = t16(n, n->a1, P, 32)) || (rc = t16(n, n->cat1, P, 32 * sk)) ||
+  // x8 (the BF16 input) and a1 (the first convolution) only for training
+  const long Pt = n->train ? P : 1;
+  if ((rc = t16(n, n->x8, Pt, 8)) || (rc = t16(n, n->a1, Pt, 32)) || (rc = t16(n, n->cat1, P, 32 * sk)) ||
       (rc = t16(n, n->p1, P2, 32)) || (rc = t16(n, n->a2, P2, 64)) || (rc = t16(n, n->cat2, P2, 64 * sk)) ||
       (rc = t16(n, n->p2, P4, 64)) || (rc = t16(n, n->a3, P4, 128)) || (rc = t16(n, n->cat3, P4, 128 * sk)) ||
       (rc = t16(n, n->p3, P8, 128)) || (rc = t16(n, n->ab, P8, 256)) || (rc = t16(n, n->bb, P8, 256)) ||

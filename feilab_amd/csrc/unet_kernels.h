@@ -37,6 +37,18 @@ struct ConvArgs {
   const float* head_w;
   const float* head_b;
   float* head_out;
+  // optional (BF16 input and output, Cin = Cout = 32, 3x3, no up): x is not
+  // read but computed per tile as the network's first convolution of the
+  // one-channel FP32 images x1: x[p][c] = relu(b1[c] + sum_t w1[t][c][0]
+  // img[p + off(t)]), img = bf16(x1 / x1_scale[image]) (x1_scale optional)
+  // and zero outside the image, w1 [9][32][w1_cin] BF16; written to a1_out
+  // (32 channels per pixel) when given
+  const float* x1;
+  const float* x1_scale;
+  const uint16_t* w1;
+  int w1_cin;
+  const float* b1;
+  uint16_t* a1_out;
 };
 
 // dw[tap][co][ci] += sum over pixels of dy[p][co] * x[p + offset(tap)][ci]
@@ -91,9 +103,6 @@ hipError_t launch_mse(const float* pred, const float* target, long P, float* dpr
 // scale[b] = RMS of image b (1 if zero); part holds B * image_rms_parts() doubles
 hipError_t launch_image_rms(const float* in, long P, int B, double* part, float* scale, hipStream_t st);
 int image_rms_parts();
-// First 3x3 convolution of a one-channel input (channel 0 of ldx channels), ReLU
-hipError_t launch_conv_first(const uint16_t* x, int ldx, const uint16_t* w, int Cin_w, const float* bias, uint16_t* y,
-                             int ldy, int N, int H, int W, int Cout, hipStream_t st);
 hipError_t launch_to_bf16_scaled(const float* in, const float* scale, long P, int B, uint16_t* out, hipStream_t st);
 hipError_t launch_residual_add(const float* m, const float* y, const float* scale, long P, int B, float* out,
                                hipStream_t st);

@@ -121,7 +121,7 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
 __device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
-template <typename TX, typename TY, int COB, int R>
+template <typename TX, typename TY, int COB, int R, bool FIRST>
 __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvArgs<TX, TY> a) {
   constexpr int CT = COB / 16;
   constexpr int CL = COB / 4;                        // consecutive output channels per lane
@@ -129,11 +129,16 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   // COB 32 with BF16 input: the next chunk's weights are prefetched into
   // registers with its input; otherwise (no registers to spare) they are
   // loaded when stored, all pieces in flight at once
-  constexpr bool kWPrefetch = COB == 32 && PW == 1;
+  // FIRST: the input is computed from the FP32 images (a single chunk whose
+  // weights are staged once, no prefetch)
+  constexpr bool kWPrefetch = COB == 32 && PW == 1 && !FIRST;
   constexpr int kTH = 4 * R;       // tile rows: R per wave
   constexpr int kHY = kTH + 2;     // halo rows
+  // FIRST: the BF16 image over the halo's own halo (kRY x kRX pixels)
+  constexpr int kRY = FIRST ? kHY + 2 : 1, kRX = kHalo + 2, kRawPer = (kRY * kRX + 255) / 256;
   __shared__ __align__(16) uint16_t s_in[kHY * kHalo * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
+  __shared__ __align__(16) uint16_t s_raw[kRY * kRX];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
@@ -191,7 +196,7 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   const int wco = co0 + slot_channel<COB>((16 * wave + lx) % COB);
   const unsigned lo_w = wco < a.Cout ? (unsigned)((wco * a.Cin + lc8) * 2) : kOutside;
   const int la_w = (16 * wave + lx) * kLdsRow + lc8;
-  uint4 pin[kRounds + kExR][PW];
+  uint4 pin[FIRST ? 1 : kRounds + kExR][PW];
   uint4 wreg[kWRounds];
   // loads the chunk's weights into registers
   const uint16_t* const Wt0 = a.w;
@@ -218,9 +223,43 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   const TX* const X = a.x;
   const uint16_t* const UP = a.up;
   const int H = a.H, W = a.W, H2 = a.H >> 1, W2 = a.W >> 1, ldx = a.ldx, ldu = a.ldu;
+  // FIRST: this thread's image pixels p = tid + 256 i of the kRY x kRX
+  // region at image (y0 - 2, x0 - 2), the first convolution's weights as
+  // MFMA A operands (taps in K positions 0..8) and its bias, per lane as
+  // the convolution's own weights and bias
+  float rawv[FIRST ? kRawPer : 1];
+  bf8 w1a[FIRST ? CT : 1];
+  float b1v[FIRST ? CL : 1];
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int j = 0; j < CT; j++) {
+      const int ch = slot_channel<COB>(16 * j + r16);
+      V8 v;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int t = 8 * g + k;
+        v.h[k] = t < 9 ? a.w1[((long)t * COB + ch) * a.w1_cin] : (uint16_t)0;
+      }
+      w1a[j] = v.v;
+    }
+#pragma unroll
+    for (int k = 0; k < CL; k++) b1v[k] = a.b1[cl0 + k];
+  }
   auto fetch = [&](int tile, int ck) {
     const int tx = tile % tiles_x, rest = tile / tiles_x;
     const int y0 = (rest % tiles_y) * kTH, x0 = tx * kTile, n = rest / tiles_y;
+    if constexpr (FIRST) {
+      const auto rs = image_rsrc(a.x1 + (long)n * a.H * a.W, 0x7fffffffu);
+#pragma unroll
+      for (int i = 0; i < kRawPer; i++) {
+        const int p = tid + 256 * i, ry = p / kRX, rx = p - kRX * ry;
+        const int yy = y0 - 2 + ry, xx = x0 - 2 + rx;
+        const bool ok = p < kRY * kRX && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+        rawv[i] = __uint_as_float(
+            __builtin_amdgcn_raw_buffer_load_b32(rs, ok ? (unsigned)((yy * a.W + xx) * 4) : kOutside, 0, 0));
+      }
+      return;
+    }
     const int ci0 = 32 * ck;
     bool from_up = false;
     if constexpr (sizeof(TX) == 2) from_up = ci0 < a.Cup;   // Cup is a multiple of 32
@@ -269,9 +308,56 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
             __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? off + 16 * h : kOutside, 0, 0));
     }
   };
+  // FIRST: the staged input is the first convolution over the halo, from the
+  // image region in LDS: per 16 halo pixels one MFMA per 16 channels, K =
+  // the 9 taps (lane group 0 taps 0..7, group 1 tap 8, the rest 0); zero
+  // outside the image (the input's padding); the tile's own pixels also to
+  // a1_out
+  auto put_first = [&](int tile) {
+    const int tx = tile % tiles_x, rest = tile / tiles_x;
+    const int y0 = (rest % tiles_y) * kTH, x0 = tx * kTile, n = rest / tiles_y;
+    const float sc = a.x1_scale ? a.x1_scale[n] : 1.f;
+#pragma unroll
+    for (int i = 0; i < kRawPer; i++) {
+      const int p = tid + 256 * i;
+      if (p < kRY * kRX) s_raw[p] = f2bf(rawv[i] / sc);   // as the input conversion rounds it
+    }
+    __syncthreads();
+    constexpr int kHP = kHY * kHalo, kPT = (kHP + 15) / 16;
+#pragma unroll 1
+    for (int pt = wv; pt < kPT; pt += 4) {
+      const int q = 16 * pt + r16;
+      const bool valid = q < kHP;
+      const int hy = q / kHalo, hx = q - kHalo * hy;
+      const int rb = hy * kRX + hx;   // image pixel of tap 0
+      V8 b;
+      b.u = make_uint4(0, 0, 0, 0);
+      if (valid && g == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) b.h[k] = s_raw[rb + (k / 3) * kRX + k % 3];
+      } else if (valid && g == 1) {
+        b.h[0] = s_raw[rb + 2 * kRX + 2];
+      }
+      f4 d[CT];
+#pragma unroll
+      for (int j = 0; j < CT; j++) d[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1a[j], b.v, f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      // channels cl0 + k = 8 g + k of halo pixel q
+      const int yy = y0 - 1 + hy, xx = x0 - 1 + hx;
+      const bool inside = valid && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+      uint4 o;
+      uint32_t* po = reinterpret_cast<uint32_t*>(&o);
+#pragma unroll
+      for (int k = 0; k < CL; k += 2)
+        po[k / 2] = inside ? pack_bf16(fmaxf(d[k >> 2][k & 3] + b1v[k], 0.f), fmaxf(d[(k + 1) >> 2][(k + 1) & 3] + b1v[k + 1], 0.f))
+                           : 0u;
+      if (valid) *reinterpret_cast<uint4*>(&s_in[(hy * kHalo + hx) * kLdsRow + 8 * g]) = o;
+      if (a.a1_out && inside && hy >= 1 && hy <= kTH && hx >= 1 && hx <= kTile)
+        *reinterpret_cast<uint4*>(a.a1_out + (((long)n * a.H + yy) * a.W + xx) * COB + cl0) = o;
+    }
+  };
   // stores the prefetched input and, when they change, the chunk's weights
   // (read straight from L2: every block of the launch reads the same ones)
-  auto put = [&](int ck, bool with_w) {
+  auto put = [&](int tile, int ck, bool with_w) {
     auto bf = [&](const uint4* p) {
       if constexpr (PW == 1) {
         return p[0];
@@ -285,14 +371,19 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
         return v;
       }
     };
+    if constexpr (FIRST) {
+      put_first(tile);
+    } else {
 #pragma unroll
-    for (int r = 0; r < kRounds; r++)
-      if (r < kRounds - 1 || wv + 4 * r < kHY)
-        *reinterpret_cast<uint4*>(&s_in[la_in + 4 * r * kHalo * kLdsRow]) = bf(pin[r]);
+      for (int r = 0; r < kRounds; r++)
+        if (r < kRounds - 1 || wv + 4 * r < kHY)
+          *reinterpret_cast<uint4*>(&s_in[la_in + 4 * r * kHalo * kLdsRow]) = bf(pin[r]);
 #pragma unroll
-    for (int e = 0; e < kExR; e++)
-      if (tid + 256 * e < 8 * kHY) *reinterpret_cast<uint4*>(&s_in[la_ex + 32 * e * kHalo * kLdsRow]) = bf(pin[kRounds + e]);
-    vm_wait_all();
+      for (int e = 0; e < kExR; e++)
+        if (tid + 256 * e < 8 * kHY)
+          *reinterpret_cast<uint4*>(&s_in[la_ex + 32 * e * kHalo * kLdsRow]) = bf(pin[kRounds + e]);
+      vm_wait_all();
+    }
     if (with_w) {
       if constexpr (!kWPrefetch) wfetch(ck);
       wput();
@@ -308,7 +399,7 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   fetch(tt, 0);
   if constexpr (kWPrefetch) wfetch(0);
   for (;;) {
-    put(ck, w_in);
+    put(tt, ck, w_in);
     __syncthreads();
     // the next work item; its loads are in flight during this item's MFMAs
     int tt2 = tt, ck2 = ck + 1;
@@ -651,52 +742,6 @@ __global__ void k_head_fwd(const uint16_t* x, int ldx, const float* w, const flo
   }
 }
 
-// First convolution, whose input has one real channel (channel 0 of an
-// ldx-channel bf16 tensor): y[p][co] = relu(bias[co] + sum_t w[t][co][0] x[p + off(t)][0]),
-// 8 output channels per 16-byte store.  Memory-bound, so plain FP32 FMAs.
-__global__ __launch_bounds__(256) void k_conv_first(const uint16_t* x, int ldx, const uint16_t* w, int Cin_w,
-                                                   const float* bias, uint16_t* y, int ldy, int N, int H, int W,
-                                                   int Cout) {
-  __shared__ float s_w[9 * 64];
-  __shared__ float s_b[64];
-  for (int i = threadIdx.x; i < 9 * Cout; i += blockDim.x) s_w[i] = bf2f(w[(long)i * Cin_w]);
-  for (int i = threadIdx.x; i < Cout; i += blockDim.x) s_b[i] = bias[i];
-  __syncthreads();
-  const long P = (long)N * H * W;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
-    const int px = (int)(p % W), py = (int)((p / W) % H);
-    const long pn = p / ((long)W * H);
-    // the 9 taps are loaded unconditionally (the pixel itself where a tap
-    // falls outside the image, the value then dropped) and taken by an empty
-    // asm before any is used: a load under a per-lane condition would be
-    // compiled into a branch whose merge waits for it before the next load
-    uint32_t raw[9];
-    bool ok[9];
-#pragma unroll
-    for (int t = 0; t < 9; t++) {
-      const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-      ok[t] = yy >= 0 && yy < H && xx >= 0 && xx < W;
-      raw[t] = x[((pn * H + (ok[t] ? yy : py)) * W + (ok[t] ? xx : px)) * ldx];
-    }
-#pragma unroll
-    for (int t = 0; t < 9; t++) asm volatile("" : "+v"(raw[t]));
-    float v[9];
-#pragma unroll
-    for (int t = 0; t < 9; t++) v[t] = ok[t] ? bf2f((uint16_t)raw[t]) : 0.f;
-    for (int c = 0; c < Cout; c += 8) {
-      V8 o;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        float acc = 0.f;
-#pragma unroll
-        for (int t = 0; t < 9; t++) acc = fmaf(v[t], s_w[t * Cout + c + k], acc);
-        o.h[k] = f2bf(fmaxf(acc + s_b[c + k], 0.f));
-      }
-      *reinterpret_cast<uint4*>(y + p * ldy + c) = o.u;
-    }
-  }
-}
-
 // head backward: dX[p][c] = dout[p] w[c] (masked by ReLU of x); dw, db reduced
 __global__ __launch_bounds__(256) void k_head_bwd(const uint16_t* x, int ldx, const float* w, int C, long P,
                                                  const float* dout, float* dx, int lddx, float* dw, float* db) {
@@ -831,15 +876,15 @@ static int blocks_for(long n, int per = 256) { return (int)std::min<long>((n + p
 // 4 kRows32 x 16 pixels); the others use 4 (their registers are full).
 constexpr int kRows32 = 8;
 
-template <typename TX, typename TY, int COB, int R>
+template <typename TX, typename TY, int COB, int R, bool FIRST = false>
 void launch_conv_tiles(const ConvArgs<TX, TY>& a, int cus, hipStream_t st) {
   // LDS per block: the staged halo tile and one chunk's weights
-  constexpr long lds = ((4L * R + 2) * kHalo + 9L * COB) * kLdsRow * 2;
+  constexpr long lds = ((4L * R + 2) * kHalo + 9L * COB) * kLdsRow * 2 + (FIRST ? (4L * R + 4) * (kHalo + 2) * 2 : 0);
   constexpr long per_cu = std::min<long>(COB == 32 && R == 4 ? 3 : 2, (160L * 1024) / lds);
   const long tiles = (long)a.N * ((a.H + 4 * R - 1) / (4 * R)) * ((a.W + kTile - 1) / kTile);
   const int cb = (a.Cout + COB - 1) / COB;
   const long gx = std::max<long>(1, std::min<long>(tiles, per_cu * cus / cb));
-  hipLaunchKernelGGL((k_conv<TX, TY, COB, R>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((k_conv<TX, TY, COB, R, FIRST>), dim3((unsigned)gx, (unsigned)cb), dim3(256), 0, st, a);
 }
 
 template <typename TX, typename TY>
@@ -867,10 +912,17 @@ hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 256;
   }
-  if (a.Cout <= 32)
+  if (a.x1) {
+    // the input computed from the images: one 32-channel chunk, plain 3x3
+    if (sizeof(TX) != 2 || sizeof(TY) != 2 || a.Cin != 32 || a.Cout != 32 || a.taps != 9 || a.up || !a.w1 ||
+        !a.b1 || a.w1_cin < 1 || (long)a.N * a.H * a.W * 4 >= (1L << 31))
+      return hipErrorInvalidValue;
+    if constexpr (sizeof(TX) == 2 && sizeof(TY) == 2) launch_conv_tiles<TX, TY, 32, kRows32, true>(a, cus, st);
+  } else if (a.Cout <= 32) {
     launch_conv_tiles<TX, TY, 32, sizeof(TX) == 2 ? kRows32 : 4>(a, cus, st);
-  else
+  } else {
     launch_conv_tiles<TX, TY, 64, 4>(a, cus, st);
+  }
   return hipGetLastError();
 }
 template hipError_t launch_conv<uint16_t, uint16_t>(const ConvArgs<uint16_t, uint16_t>&, hipStream_t);
@@ -940,13 +992,6 @@ hipError_t launch_image_rms(const float* in, long P, int B, double* part, float*
   return hipGetLastError();
 }
 int image_rms_parts() { return kRmsParts; }
-hipError_t launch_conv_first(const uint16_t* x, int ldx, const uint16_t* w, int Cin_w, const float* bias, uint16_t* y,
-                             int ldy, int N, int H, int W, int Cout, hipStream_t st) {
-  if (Cout > 64 || Cout % 8) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_conv_first, dim3(blocks_for((long)N * H * W)), dim3(256), 0, st, x, ldx, w, Cin_w, bias, y,
-                     ldy, N, H, W, Cout);
-  return hipGetLastError();
-}
 hipError_t launch_to_bf16_scaled(const float* in, const float* scale, long P, int B, uint16_t* out, hipStream_t st) {
   hipLaunchKernelGGL(k_to_bf16_scaled, dim3(blocks_for(P * B)), dim3(256), 0, st, in, scale, P, P * B, out);
   return hipGetLastError();
