@@ -109,15 +109,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t image_rsrc(const void* p, unsi
 __device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // two floats rounded to BF16 (nearest even, as f2bf) in one dword, lo first
+// (one v_cvt_pk_bf16_f32)
+typedef __attribute__((ext_vector_type(2))) float f2v;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf2v;
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
-  union {
-    __bf16 h[2];
-    uint32_t u;
-  } r;
-  r.h[0] = (__bf16)lo;
-  r.h[1] = (__bf16)hi;
-  return r.u;
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){lo, hi}, bf2v));
 }
+// max(v, 0) as an integer max of the bits (one instruction; the float max
+// is preceded by a NaN-quieting copy): equal for every non-NaN v
+__device__ __forceinline__ float relu(float v) { return __int_as_float(max(__float_as_int(v), 0)); }
 __device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
@@ -134,11 +134,13 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   constexpr bool kWPrefetch = COB == 32 && PW == 1 && !FIRST;
   constexpr int kTH = 4 * R;       // tile rows: R per wave
   constexpr int kHY = kTH + 2;     // halo rows
-  // FIRST: the BF16 image over the halo's own halo (kRY x kRX pixels)
+  // FIRST: the BF16 image over the halo's own halo (kRY x kRX pixels), per
+  // pixel its row of three taps (the pixel and the next two) in 8 bytes;
+  // one more zero entry
   constexpr int kRY = FIRST ? kHY + 2 : 1, kRX = kHalo + 2, kRawPer = (kRY * kRX + 255) / 256;
   __shared__ __align__(16) uint16_t s_in[kHY * kHalo * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
-  __shared__ __align__(16) uint16_t s_raw[kRY * kRX];
+  __shared__ __align__(16) uint2 s_raw3[FIRST ? kRY * kRX + 1 : 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r16 = lane & 15, g = lane >> 4;
@@ -224,12 +226,13 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   const uint16_t* const UP = a.up;
   const int H = a.H, W = a.W, H2 = a.H >> 1, W2 = a.W >> 1, ldx = a.ldx, ldu = a.ldu;
   // FIRST: this thread's image pixels p = tid + 256 i of the kRY x kRX
-  // region at image (y0 - 2, x0 - 2), the first convolution's weights as
-  // MFMA A operands (taps in K positions 0..8) and its bias, per lane as
-  // the convolution's own weights and bias
-  float rawv[FIRST ? kRawPer : 1];
+  // region at image (y0 - 2, x0 - 2) and the two after each; the first
+  // convolution's weights as MFMA A operands, K position 4 ty + tx for tap
+  // (ty, tx) (positions 4 ty + 3 and from 12 on zero), and its bias as the
+  // MFMA's C operand, per lane as the second convolution's own slots
+  float rawv[FIRST ? kRawPer : 1][3];
   bf8 w1a[FIRST ? CT : 1];
-  float b1v[FIRST ? CL : 1];
+  f4 b1c[FIRST ? CT : 1];
   if constexpr (FIRST) {
 #pragma unroll
     for (int j = 0; j < CT; j++) {
@@ -237,13 +240,14 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
       V8 v;
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const int t = 8 * g + k;
-        v.h[k] = t < 9 ? a.w1[((long)t * COB + ch) * a.w1_cin] : (uint16_t)0;
+        const int ty = 2 * g + k / 4, tx = k % 4;
+        v.h[k] = tx < 3 && ty < 3 ? a.w1[((long)(3 * ty + tx) * COB + ch) * a.w1_cin] : (uint16_t)0;
       }
       w1a[j] = v.v;
+      // result rows 4 g + r of weight tile j: channels cl0 + 4 j + r
+      b1c[j] = f4{a.b1[cl0 + 4 * j], a.b1[cl0 + 4 * j + 1], a.b1[cl0 + 4 * j + 2], a.b1[cl0 + 4 * j + 3]};
     }
-#pragma unroll
-    for (int k = 0; k < CL; k++) b1v[k] = a.b1[cl0 + k];
+    if (tid == 0) s_raw3[kRY * kRX] = make_uint2(0, 0);
   }
   auto fetch = [&](int tile, int ck) {
     const int tx = tile % tiles_x, rest = tile / tiles_x;
@@ -254,9 +258,13 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
       for (int i = 0; i < kRawPer; i++) {
         const int p = tid + 256 * i, ry = p / kRX, rx = p - kRX * ry;
         const int yy = y0 - 2 + ry, xx = x0 - 2 + rx;
-        const bool ok = p < kRY * kRX && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
-        rawv[i] = __uint_as_float(
-            __builtin_amdgcn_raw_buffer_load_b32(rs, ok ? (unsigned)((yy * a.W + xx) * 4) : kOutside, 0, 0));
+        const bool oky = p < kRY * kRX && (unsigned)yy < (unsigned)a.H;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const bool ok = oky && (unsigned)(xx + d) < (unsigned)a.W;
+          rawv[i][d] = __uint_as_float(
+              __builtin_amdgcn_raw_buffer_load_b32(rs, ok ? (unsigned)((yy * a.W + xx + d) * 4) : kOutside, 0, 0));
+        }
       }
       return;
     }
@@ -309,8 +317,8 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
     }
   };
   // FIRST: the staged input is the first convolution over the halo, from the
-  // image region in LDS: per 16 halo pixels one MFMA per 16 channels, K =
-  // the 9 taps (lane group 0 taps 0..7, group 1 tap 8, the rest 0); zero
+  // image rows in LDS: per 16 halo pixels one MFMA per 16 channels, K = the
+  // taps (lane group 0 tap rows 0 and 1, group 1 row 2, the rest zero); zero
   // outside the image (the input's padding); the tile's own pixels also to
   // a1_out
   auto put_first = [&](int tile) {
@@ -320,39 +328,47 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
 #pragma unroll
     for (int i = 0; i < kRawPer; i++) {
       const int p = tid + 256 * i;
-      if (p < kRY * kRX) s_raw[p] = f2bf(rawv[i] / sc);   // as the input conversion rounds it
+      // as the input conversion rounds the image
+      if (p < kRY * kRX)
+        s_raw3[p] = make_uint2((uint32_t)f2bf(rawv[i][0] / sc) | ((uint32_t)f2bf(rawv[i][1] / sc) << 16),
+                               (uint32_t)f2bf(rawv[i][2] / sc));
     }
     __syncthreads();
-    constexpr int kHP = kHY * kHalo, kPT = (kHP + 15) / 16;
+    constexpr int kHP = kHY * kHalo, kPT = (kHP + 15) / 16, kZero = kRY * kRX;
+    // every halo pixel inside the image (most tiles): no per-pixel test
+    const bool interior = y0 >= 1 && y0 + kTH + 1 <= a.H && x0 >= 1 && x0 + kTile + 1 <= a.W;
+    // halo pixel q = 16 pt + r16, pt = wv, wv + 4, ...: q grows by 64 = 3 rows + 10 columns
+    int hy = (16 * wv + r16) / kHalo, hx = (16 * wv + r16) - kHalo * hy;
 #pragma unroll 1
     for (int pt = wv; pt < kPT; pt += 4) {
-      const int q = 16 * pt + r16;
-      const bool valid = q < kHP;
-      const int hy = q / kHalo, hx = q - kHalo * hy;
-      const int rb = hy * kRX + hx;   // image pixel of tap 0
+      const bool valid = 16 * pt + r16 < kHP;
+      const int rb = hy * kRX + hx;   // image pixel of tap (0, 0)
+      const int i0 = !valid ? kZero : g == 0 ? rb : g == 1 ? rb + 2 * kRX : kZero;
+      const int i1 = valid && g == 0 ? rb + kRX : kZero;
+      const uint2 lo = s_raw3[i0], hi = s_raw3[i1];
       V8 b;
-      b.u = make_uint4(0, 0, 0, 0);
-      if (valid && g == 0) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) b.h[k] = s_raw[rb + (k / 3) * kRX + k % 3];
-      } else if (valid && g == 1) {
-        b.h[0] = s_raw[rb + 2 * kRX + 2];
-      }
+      b.u = make_uint4(lo.x, lo.y, hi.x, hi.y);
       f4 d[CT];
 #pragma unroll
-      for (int j = 0; j < CT; j++) d[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1a[j], b.v, f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-      // channels cl0 + k = 8 g + k of halo pixel q
+      for (int j = 0; j < CT; j++) d[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1a[j], b.v, b1c[j], 0, 0, 0);
+      // channels cl0 + k = 8 g + k of halo pixel (hy, hx)
       const int yy = y0 - 1 + hy, xx = x0 - 1 + hx;
-      const bool inside = valid && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+      const bool inside = valid && (interior || ((unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W));
       uint4 o;
       uint32_t* po = reinterpret_cast<uint32_t*>(&o);
 #pragma unroll
       for (int k = 0; k < CL; k += 2)
-        po[k / 2] = inside ? pack_bf16(fmaxf(d[k >> 2][k & 3] + b1v[k], 0.f), fmaxf(d[(k + 1) >> 2][(k + 1) & 3] + b1v[k + 1], 0.f))
-                           : 0u;
+        po[k / 2] = pack_bf16(relu(d[k >> 2][k & 3]), relu(d[(k + 1) >> 2][(k + 1) & 3]));
+      if (!inside) o = make_uint4(0, 0, 0, 0);
       if (valid) *reinterpret_cast<uint4*>(&s_in[(hy * kHalo + hx) * kLdsRow + 8 * g]) = o;
       if (a.a1_out && inside && hy >= 1 && hy <= kTH && hx >= 1 && hx <= kTile)
         *reinterpret_cast<uint4*>(a.a1_out + (((long)n * a.H + yy) * a.W + xx) * COB + cl0) = o;
+      hx += 10;
+      hy += 3;
+      if (hx >= kHalo) {
+        hx -= kHalo;
+        hy += 1;
+      }
     }
   };
   // stores the prefetched input and, when they change, the chunk's weights
@@ -480,8 +496,8 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
               float v0 = acc[i][k >> 2][k & 3] + bl[k];
               float v1 = acc[i][(k + 1) >> 2][(k + 1) & 3] + bl[k + 1];
               if (a.relu) {
-                v0 = fmaxf(v0, 0.f);
-                v1 = fmaxf(v1, 0.f);
+                v0 = relu(v0);
+                v1 = relu(v1);
               }
               pk[h][k / 2] = pack_bf16(v0, v1);
             }
@@ -548,7 +564,7 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
             const int co = cl0 + k;
             if (co >= a.Cout) continue;
             float v = acc[i][k >> 2][k & 3] + bl[k];
-            if (a.relu) v = fmaxf(v, 0.f);
+            if (a.relu) v = relu(v);
             if (a.mask && !(bf2f(a.mask[p * a.ldm + a.cm0 + co]) > 0.f)) v = 0.f;
             TY* dst = a.y + p * a.ldy + a.cy0 + co;
             if (a.accumulate) v += load1(dst);
