@@ -58,13 +58,13 @@ __device__ __forceinline__ float load1(const float* p) { return *p; }
 __device__ __forceinline__ void store1(float* p, float v) { *p = v; }
 
 // ----------------------------------------------------------------- conv
-// Block: 256 threads = 4 waves on a 16 x 16 pixel tile of one image and COB
+// Block: 256 threads = 4 waves on a 4R x 16 pixel tile of one image and COB
 // output channels.  Per 32-channel chunk of the input the block stages the
-// tile with its one-pixel halo (18 x 18 pixels) and the chunk's weights for
-// all taps in LDS (rows padded to 80 bytes, so the 16 rows a 16-lane group
-// reads with ds_read_b128 fall in distinct banks); wave w then multiplies the
-// COB/16 weight tiles by tile rows 4w..4w+3 (4 MFMA column tiles of 16
-// pixels) for every tap, reading both operands from LDS.
+// tile with its one-pixel halo ((4R + 2) x 18 pixels) and the chunk's
+// weights for all taps in LDS (64-byte rows with a bank swizzle, kLdsRow);
+// wave w then multiplies the COB/16 weight tiles by tile rows Rw..Rw+R-1 (R
+// MFMA column tiles of 16 pixels) for every tap, reading both operands from
+// LDS.
 //
 // The weights are the MFMA's A operand and the pixels its B operand: lane l
 // receives rows 4(l/16)..4(l/16)+3 of every weight tile's result for pixel
@@ -85,7 +85,17 @@ __device__ __forceinline__ void store1(float* p, float v) { *p = v; }
 // given an offset past the image's end, which the buffer load returns as 0.
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
-constexpr int kLdsRow = 40;   // bf16 per staged pixel / weight row (32 + 8 pad)
+// LDS rows of staged pixels and weights: 64 bytes (32 BF16), the 16-byte
+// piece c of LDS row p stored at position c ^ swz(p), swz(p) = (p / 2) % 4.
+// The operand reads and the staging writes (ds_read_b128, ds_write_b128)
+// take 16 consecutive rows x pieces g = l / 16 per wavefront; measured on
+// gfx950, plain 64- or 80-byte rows read at half the rate of consecutive
+// memory (80-byte rows write at full rate), 64-byte rows with this swizzle
+// read and write at full rate.  A staged halo row is kPitch = 24 LDS rows
+// (18 used), so that swz of a pixel depends on its column only.
+constexpr int kLdsRow = 32;
+constexpr int kPitch = 24;
+__device__ __forceinline__ int swz(int x) { return (x >> 1) & 3; }
 constexpr unsigned kOutside = 0x80000000u;         // a byte offset past every image
 
 // Output channel (within the block's COB) of result slot s = 16 j + m, m the
@@ -126,19 +136,19 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   constexpr int CT = COB / 16;
   constexpr int CL = COB / 4;                        // consecutive output channels per lane
   constexpr int PW = sizeof(TX) / 2;                 // 16-byte loads per input piece
-  // COB 32 with BF16 input: the next chunk's weights are prefetched into
-  // registers with its input; otherwise (no registers to spare) they are
-  // loaded when stored, all pieces in flight at once
+  // BF16 input: the next chunk's weights are prefetched into registers with
+  // its input; FP32 input (no registers to spare): they are loaded when
+  // stored, all pieces in flight at once
   // FIRST: the input is computed from the FP32 images (a single chunk whose
   // weights are staged once, no prefetch)
-  constexpr bool kWPrefetch = COB == 32 && PW == 1 && !FIRST;
+  constexpr bool kWPrefetch = PW == 1 && !FIRST;
   constexpr int kTH = 4 * R;       // tile rows: R per wave
   constexpr int kHY = kTH + 2;     // halo rows
   // FIRST: the BF16 image over the halo's own halo (kRY x kRX pixels), per
   // pixel its row of three taps (the pixel and the next two) in 8 bytes;
   // one more zero entry
   constexpr int kRY = FIRST ? kHY + 2 : 1, kRX = kHalo + 2, kRawPer = (kRY * kRX + 255) / 256;
-  __shared__ __align__(16) uint16_t s_in[kHY * kHalo * kLdsRow];
+  __shared__ __align__(16) uint16_t s_in[kHY * kPitch * kLdsRow];
   __shared__ __align__(16) uint16_t s_w[9 * COB * kLdsRow];
   __shared__ __align__(16) uint2 s_raw3[FIRST ? kRY * kRX + 1 : 1];
   const int tid = threadIdx.x;
@@ -168,8 +178,8 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   }
   // ---- staging.  One wave instruction moves 16 consecutive pixels (or
   // weight rows) x 4 pieces of 8 channels: lane l takes pixel / row l % 16
-  // and piece (l / 16) % 4, so the 16 lanes of one piece write 16
-  // consecutive 80-byte LDS rows (distinct banks) and the wave reads 1 KB of
+  // and piece (l / 16) % 4, so each 32 lanes write 16 consecutive LDS rows
+  // x 2 pieces (distinct banks, kLdsRow) and the wave reads 1 KB of
   // consecutive memory.  Offsets that do not change during the launch are
   // computed here once; a piece outside the image or the tensor is given
   // the offset kOutside, which the buffer load returns as 0.
@@ -189,15 +199,15 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
   const int ex_x = kTile + ((tid >> 2) & 1), ex_c8 = 8 * (tid & 3);
   // LDS element offsets of this lane's pieces (main: round 0; round r is
   // 4 r halo rows further)
-  const int la_in = (wave * kHalo + lx) * kLdsRow + lc8;
-  const int la_ex = ((tid >> 3) * kHalo + ex_x) * kLdsRow + ex_c8;   // round e: + 32 e halo rows
+  const int la_in = (wave * kPitch + lx) * kLdsRow + 8 * (((lane >> 4) & 3) ^ swz(lx));
+  const int la_ex = ((tid >> 3) * kPitch + ex_x) * kLdsRow + 8 * ((tid & 3) ^ swz(ex_x));   // round e: + 32 e halo rows
   // Weights: slot row r = tap * COB + slot; round k covers rows 64 k ... 64 k
   // + 63, wave w the rows 64 k + 16 w + (0..15), so the tap (64 k + 16 w) /
   // COB of a round is uniform and the lane's slot (16 w + lx) % COB fixed.
   constexpr int kWRounds = (9 * COB + 63) / 64;
   const int wco = co0 + slot_channel<COB>((16 * wave + lx) % COB);
   const unsigned lo_w = wco < a.Cout ? (unsigned)((wco * a.Cin + lc8) * 2) : kOutside;
-  const int la_w = (16 * wave + lx) * kLdsRow + lc8;
+  const int la_w = (16 * wave + lx) * kLdsRow + 8 * (((lane >> 4) & 3) ^ swz(lx));
   uint4 pin[FIRST ? 1 : kRounds + kExR][PW];
   uint4 wreg[kWRounds];
   // loads the chunk's weights into registers
@@ -360,7 +370,7 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
       for (int k = 0; k < CL; k += 2)
         po[k / 2] = pack_bf16(relu(d[k >> 2][k & 3]), relu(d[(k + 1) >> 2][(k + 1) & 3]));
       if (!inside) o = make_uint4(0, 0, 0, 0);
-      if (valid) *reinterpret_cast<uint4*>(&s_in[(hy * kHalo + hx) * kLdsRow + 8 * g]) = o;
+      if (valid) *reinterpret_cast<uint4*>(&s_in[(hy * kPitch + hx) * kLdsRow + 8 * (g ^ swz(hx))]) = o;
       if (a.a1_out && inside && hy >= 1 && hy <= kTH && hx >= 1 && hx <= kTile)
         *reinterpret_cast<uint4*>(a.a1_out + (((long)n * a.H + yy) * a.W + xx) * COB + cl0) = o;
       hx += 10;
@@ -393,11 +403,11 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
 #pragma unroll
       for (int r = 0; r < kRounds; r++)
         if (r < kRounds - 1 || wv + 4 * r < kHY)
-          *reinterpret_cast<uint4*>(&s_in[la_in + 4 * r * kHalo * kLdsRow]) = bf(pin[r]);
+          *reinterpret_cast<uint4*>(&s_in[la_in + 4 * r * kPitch * kLdsRow]) = bf(pin[r]);
 #pragma unroll
       for (int e = 0; e < kExR; e++)
         if (tid + 256 * e < 8 * kHY)
-          *reinterpret_cast<uint4*>(&s_in[la_ex + 32 * e * kHalo * kLdsRow]) = bf(pin[kRounds + e]);
+          *reinterpret_cast<uint4*>(&s_in[la_ex + 32 * e * kPitch * kLdsRow]) = bf(pin[kRounds + e]);
       vm_wait_all();
     }
     if (with_w) {
@@ -436,10 +446,11 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
 #pragma unroll 1
       for (int dx = 0; dx < 3; dx++) {
         bf8 P[R + 2];
+        const int pc = 8 * (g ^ swz(r16 + dx));
 #pragma unroll
         for (int h = 0; h < R + 2; h++) {
           V8 v;
-          v.u = *reinterpret_cast<const uint4*>(&s_in[((R * wave + h) * kHalo + r16 + dx) * kLdsRow + 8 * g]);
+          v.u = *reinterpret_cast<const uint4*>(&s_in[((R * wave + h) * kPitch + r16 + dx) * kLdsRow + pc]);
           P[h] = v.v;
         }
 #pragma unroll
@@ -449,7 +460,7 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
 #pragma unroll
           for (int j = 0; j < CT; j++) {
             V8 v;
-            v.u = *reinterpret_cast<const uint4*>(&s_w[(t * COB + 16 * j + r16) * kLdsRow + 8 * g]);
+            v.u = *reinterpret_cast<const uint4*>(&s_w[(t * COB + 16 * j + r16) * kLdsRow + 8 * (g ^ swz(r16))]);
             Wt[j] = v.v;
           }
 #pragma unroll
@@ -465,13 +476,13 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
 #pragma unroll
       for (int j = 0; j < CT; j++) {
         V8 v;
-        v.u = *reinterpret_cast<const uint4*>(&s_w[(16 * j + r16) * kLdsRow + 8 * g]);
+        v.u = *reinterpret_cast<const uint4*>(&s_w[(16 * j + r16) * kLdsRow + 8 * (g ^ swz(r16))]);
         Wt[j] = v.v;
       }
 #pragma unroll
       for (int i = 0; i < R; i++) {
         V8 v;
-        v.u = *reinterpret_cast<const uint4*>(&s_in[((R * wave + i + 1) * kHalo + r16 + 1) * kLdsRow + 8 * g]);
+        v.u = *reinterpret_cast<const uint4*>(&s_in[((R * wave + i + 1) * kPitch + r16 + 1) * kLdsRow + 8 * (g ^ swz(r16 + 1))]);
 #pragma unroll
         for (int j = 0; j < CT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wt[j], v.v, acc[i][j], 0, 0, 0);
       }
@@ -895,7 +906,7 @@ constexpr int kRows32 = 8;
 template <typename TX, typename TY, int COB, int R, bool FIRST = false>
 void launch_conv_tiles(const ConvArgs<TX, TY>& a, int cus, hipStream_t st) {
   // LDS per block: the staged halo tile and one chunk's weights
-  constexpr long lds = ((4L * R + 2) * kHalo + 9L * COB) * kLdsRow * 2 + (FIRST ? (4L * R + 4) * (kHalo + 2) * 2 : 0);
+  constexpr long lds = ((4L * R + 2) * kPitch + 9L * COB) * kLdsRow * 2 + (FIRST ? ((4L * R + 4) * (kHalo + 2) + 1) * 8 : 0);
   constexpr long per_cu = std::min<long>(COB == 32 && R == 4 ? 3 : 2, (160L * 1024) / lds);
   const long tiles = (long)a.N * ((a.H + 4 * R - 1) / (4 * R)) * ((a.W + kTile - 1) / kTile);
   const int cb = (a.Cout + COB - 1) / COB;
@@ -934,7 +945,9 @@ hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st) {
         !a.b1 || a.w1_cin < 1 || (long)a.N * a.H * a.W * 4 >= (1L << 31))
       return hipErrorInvalidValue;
     if constexpr (sizeof(TX) == 2 && sizeof(TY) == 2) launch_conv_tiles<TX, TY, 32, kRows32, true>(a, cus, st);
-  } else if (a.Cout <= 32) {
+  } else if (a.Cout <= 32 || (sizeof(TX) == 2 && a.Cin >= 128)) {
+    // (BF16 layers with 4 or more input chunks: measured faster on 32-channel
+    // 32 x 16 tiles than on 64-channel 16 x 16 ones)
     launch_conv_tiles<TX, TY, 32, sizeof(TX) == 2 ? kRows32 : 4>(a, cus, st);
   } else {
     launch_conv_tiles<TX, TY, 64, 4>(a, cus, st);
