@@ -71,17 +71,21 @@ def algorithmic_bytes(plan):
         "col_fwd": grid + 8 * M,
         # spread: the node values once + the full oversampled grid once
         "col_adj": 8 * M + grid,
-        # image in (the update reads the gradient and rewrites the image) + the N0 non-zero grid rows out
-        "row_fwd": 8 * N0 * N1 + 8 * N0 * n1,
-        # N0 grid rows in + A^H W b in + gradient out + the N0 grid rows of A rhat out
-        "row_adj": 8 * N0 * n1 + 16 * N0 * N1 + 8 * N0 * n1,
+        # the update: image and Re(rhat) in, image out (f32); + the N0 non-zero grid rows out
+        "row_fwd": 12 * N0 * N1 + 8 * N0 * n1,
+        # N0 grid rows in + A^H W b in (c64) + Re(rhat) out (f32, all the update reads) + the N0 grid
+        # rows of A rhat out
+        "row_adj": 8 * N0 * n1 + 12 * N0 * N1 + 8 * N0 * n1,
         # interpolation partials in (<= 2 per node) + data + weighted values out
         "combine": 16 * M + 8 * M + 8 * M,
+        # the step size: interpolation partials of A rhat in + the row sums of |rhat|^2 (f64)
+        "combine_alpha": 16 * M + 8 * N0,
     }
 
 
-# row_adj also does the row pass of A rhat (fused), so row_fwd runs once
-LAUNCHES_PER_ITER = {"row_fwd": 1, "col_fwd": 2, "combine": 2, "col_adj": 1, "row_adj": 1}
+# row_adj also does the row pass of A rhat (fused), so row_fwd runs once; the
+# second combine is the step size's
+LAUNCHES_PER_ITER = {"row_fwd": 1, "col_fwd": 2, "combine": 1, "col_adj": 1, "row_adj": 1, "combine_alpha": 1}
 
 
 def cpu_model():

@@ -186,17 +186,20 @@ void fill_args(icon_batch* b, const float* const* sino, float* const* m, float* 
     const int S = b->S[i];
     const int E = (int)p->bins.ent.size();
     RowFwdArgs& rf = b->h_rf[i];
-    rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.m = m[i]; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
+    rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.m = m[i]; rf.grad = p->d_grad; rf.alpha = p->d_alpha;
     rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
     RowAdjArgs& ra = b->h_ra[i];
     ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.in = p->d_tmp; ra.ck = p->d_ck; ra.tw = p->d_tw1;
-    ra.img = p->d_ahwb; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart; ra.fwd_out = p->d_tmp;
+    ra.img = p->d_ahwb; ra.ahwb = p->d_ahwb; ra.grad = p->d_grad; ra.rowpart = p->d_rowpart; ra.fwd_out = p->d_tmp;
     ColArgs& ca = b->h_ca[i];
     ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.E = E; ca.K = g.K;
     ca.in = p->d_tmp; ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent;
     ca.xoff = p->d_xoff; ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.ent_px = p->d_ent_px;
     ca.ent_py = p->d_ent_py; ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
     ca.part = p->d_part; ca.fw = p->d_fw;
+    zero_band(p->bins, &rf.zlo, &rf.zhi);
+    zero_band(p->bins, &ra.zlo, &ra.zhi);
+    zero_band(p->bins, &ca.zlo, &ca.zhi);
     CombineArgs& cm = b->h_cm[i];
     cm.S = S; cm.M = g.M; cm.N0 = g.N0; cm.nz = g.nz; cm.E = E; cm.part = p->d_part; cm.nslot = p->d_nslot;
     cm.w = p->d_w; cm.b = p->d_b; cm.fw = p->d_fw; cm.ent_of_node = p->d_ent_of_node; cm.bnorm = p->d_bnorm;

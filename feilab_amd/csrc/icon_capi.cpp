@@ -184,6 +184,7 @@ int op_forward(icon_plan* p, int S, const void* img, int cplx, float2* out, hipS
   ra.m = cplx ? nullptr : (float*)img;
   ra.rhat = cplx ? (const float2*)img : nullptr;
   ra.ck = p->d_ck; ra.tw = p->d_tw1; ra.out = p->d_tmp;
+  zero_band(p->bins, &ra.zlo, &ra.zhi);
   int rc = check_launch(launch_row_fwd(g.n1, ra, st), "row_fwd");
   if (rc) return rc;
   ColArgs ca{};
@@ -209,11 +210,13 @@ int op_adjoint_ent(icon_plan* p, int S, float2* img, hipStream_t st) {
   ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.fw = f;
+  zero_band(p->bins, &ca.zlo, &ca.zhi);
   int rc = check_launch(launch_col(g.n0, p->bins.C, true, ca, st), "col_adj");
   if (rc) return rc;
   RowAdjArgs ra{};
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 0; ra.in = p->d_tmp;
   ra.ck = p->d_ck; ra.tw = p->d_tw1; ra.img = img;
+  zero_band(p->bins, &ra.zlo, &ra.zhi);
   return check_launch(launch_row_adj(g.n1, ra, st), "row_adj");
 }
 
@@ -261,7 +264,7 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
 
   RowFwdArgs rf{};
   rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.data_type = o.data_type; rf.thr = o.threshold;
-  rf.m = m; rf.rhat = p->d_rhat; rf.alpha = p->d_alpha; rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
+  rf.m = m; rf.grad = p->d_grad; rf.alpha = p->d_alpha; rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
@@ -278,8 +281,11 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   cr.alpha_stride = iters;
   RowAdjArgs ra{};
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 1; ra.in = p->d_tmp; ra.ck = p->d_ck;
-  ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart;
+  ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.grad = p->d_grad; ra.rowpart = p->d_rowpart;
   ra.fwd_out = p->d_tmp;   // the row pass of A rhat fused in
+  zero_band(p->bins, &rf.zlo, &rf.zhi);
+  zero_band(p->bins, &ca.zlo, &ca.zhi);
+  zero_band(p->bins, &ra.zlo, &ra.zhi);
   auto phase = [&](int it) { return (o.pre <= it && it < iters - o.post) ? 2 : 1; };
 
   for (int it = 0; it < iters; it++) {
@@ -533,7 +539,7 @@ int plan_create(icon_plan** out, int device, int nx, int thickness, int nz, cons
   if ((rc = dalloc(p, &p->d_b, S * M))) return bail(rc);
   if ((rc = dalloc(p, &p->d_part, S * M * kMaxSlot))) return bail(rc);
   if ((rc = dalloc(p, &p->d_ahwb, S * NN))) return bail(rc);
-  if ((rc = dalloc(p, &p->d_rhat, S * NN))) return bail(rc);
+  if ((rc = dalloc(p, &p->d_grad, S * NN))) return bail(rc);
   if ((rc = dalloc(p, &p->d_tmp, S * g.N0 * (size_t)g.n1))) return bail(rc);
   if ((rc = dalloc(p, &p->d_frep, S * g.N0))) return bail(rc);
   if ((rc = dalloc(p, &p->d_rowpart, S * g.N0))) return bail(rc);
@@ -564,7 +570,7 @@ int icon_plan_destroy(icon_plan* p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   drop_graph(p);
   void* ptrs[] = {p->d_ck, p->d_ck1, p->d_w, p->d_phix, p->d_phiy, p->d_ix0, p->d_phi1, p->d_i10,
-                  p->d_b, p->d_part, p->d_ahwb, p->d_rhat, p->d_tmp, p->d_frep, p->d_rowpart,
+                  p->d_b, p->d_part, p->d_ahwb, p->d_grad, p->d_tmp, p->d_frep, p->d_rowpart,
                   p->d_bnorm, p->d_alpha};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -770,7 +776,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   // the row pass with the ICON update fused, as in every iteration after the
   // first; the image it updates lives in the (then stale) A^H W b buffer
   rf.rows = S * g.N0; rf.N0 = g.N0; rf.N1 = g.N1; rf.cplx = 0; rf.update = 2; rf.data_type = 1;
-  rf.m = reinterpret_cast<float*>(p->d_ahwb); rf.rhat = p->d_rhat; rf.alpha = p->d_alpha;
+  rf.m = reinterpret_cast<float*>(p->d_ahwb); rf.grad = p->d_grad; rf.alpha = p->d_alpha;
   rf.ck = p->d_ck; rf.tw = p->d_tw1; rf.out = p->d_tmp;
   ColArgs ca{};
   ca.K = p->g.K;
@@ -784,8 +790,14 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   cm.E = (int)p->bins.ent.size(); cm.ent_of_node = p->d_ent_of_node;
   RowAdjArgs ra{};
   ra.rows = S * g.N0; ra.N0 = g.N0; ra.N1 = g.N1; ra.mode = 1; ra.in = p->d_tmp; ra.ck = p->d_ck;
-  ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.rhat = p->d_rhat; ra.rowpart = p->d_rowpart;
+  ra.tw = p->d_tw1; ra.ahwb = p->d_ahwb; ra.grad = p->d_grad; ra.rowpart = p->d_rowpart;
   ra.fwd_out = p->d_tmp;
+  zero_band(p->bins, &rf.zlo, &rf.zhi);
+  zero_band(p->bins, &ca.zlo, &ca.zhi);
+  zero_band(p->bins, &ra.zlo, &ra.zhi);
+  // the step-size combine (mode 1) after the combine of A m, as in a solve
+  CombineArgs cr = cm;
+  cr.mode = 1; cr.rowpart = p->d_rowpart; cr.alpha = p->d_alpha;
   SinoArgs sa{};
   sa.K = p->g.K;
   sa.rows = S * g.nz; sa.N0 = g.N0; sa.sino = reinterpret_cast<const float*>(p->d_b); sa.ck1 = p->d_ck1;
@@ -798,17 +810,19 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
       case ICON_K_COL_ADJ: return launch_col(g.n0, p->bins.C, true, ca, st);
       case ICON_K_ROW_ADJ: return launch_row_adj(g.n1, ra, st);
       case ICON_K_SINO: return launch_sino(g.n0, sa, st);
+      case ICON_K_COMBINE_ALPHA: return launch_combine(cr, st);
       default: return hipErrorInvalidValue;
     }
   };
-  if (kernel < ICON_K_ROW_FWD || kernel > ICON_K_SINO) return fail(ICON_ERR_ARG, "icon_time_kernel: unknown kernel %d", kernel);
+  if (kernel < ICON_K_ROW_FWD || kernel > ICON_K_COMBINE_ALPHA)
+    return fail(ICON_ERR_ARG, "icon_time_kernel: unknown kernel %d", kernel);
   // The launch sequence of one iteration, as the solve enqueues it; the
   // timed kernel runs inside it, so that each timed launch finds its input as
   // in a solve (just written by its predecessor, partly in the caches) and the
   // device busy (an idle gap before a launch lowers the clock it starts at).
   // The sinogram transform runs once per solve and is timed back to back.
   const int iter_seq[7] = {ICON_K_ROW_FWD, ICON_K_COL_FWD, ICON_K_COMBINE, ICON_K_COL_ADJ,
-                           ICON_K_ROW_ADJ, ICON_K_COL_FWD, ICON_K_COMBINE};
+                           ICON_K_ROW_ADJ, ICON_K_COL_FWD, ICON_K_COMBINE_ALPHA};
   const int sino_seq[1] = {ICON_K_SINO};
   const int* seq = kernel == ICON_K_SINO ? sino_seq : iter_seq;
   const int nseq = kernel == ICON_K_SINO ? 1 : 7;

@@ -17,11 +17,13 @@ struct RowFwdArgs {
   int data_type;
   float thr;
   float* m;
-  const float2* rhat;
+  const float2* rhat;   // cplx: the complex image
+  const float* grad;    // update: Re(rhat) [S][N0][N1]
   const double* alpha;
   const float* ck;
   const float2* tw;
   float2* out;     // [S][N0][NR]
+  int zlo, zhi;    // out columns [zlo, zhi) are not stored (no interpolation reads them)
 };
 
 struct RowAdjArgs {
@@ -32,9 +34,10 @@ struct RowAdjArgs {
   const float2* tw;
   float2* img;         // mode 0 output
   const float2* ahwb;  // mode 1
-  float2* rhat;        // mode 1 output
+  float* grad;         // mode 1 output: Re(rhat) [S][N0][N1], all the update reads
   double* rowpart;     // mode 1: sum |rhat|^2 per row
   float2* fwd_out;     // mode 1, or null: y-FFT of rhat/ck padded to NR, per row [S][N0][NR]
+  int zlo, zhi;        // columns [zlo, zhi) of in are zero (not loaded) and of fwd_out not stored
 };
 
 struct ColArgs {
@@ -57,6 +60,7 @@ struct ColArgs {
   const float* fent_py;
   float2* part;            // fwd: [S][M][3]
   const float2* fw;        // adj: [S][E] weighted value per bin entry
+  int zlo, zhi;            // adj: tiles inside columns [zlo, zhi) store nothing (their zeros are implied)
 };
 
 struct CombineArgs {

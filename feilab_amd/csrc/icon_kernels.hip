@@ -50,14 +50,14 @@ constexpr int kRowLds = RowLay::size(NR);
 // Inputs 0..3 and 12..15 of thread t's first pass over image row X of slice
 // s (the others are padding): CPLX, the complex image rhat; otherwise the
 // real image m, with the ICON update m <- (float)(m - alpha Re(rhat)) and its
-// threshold applied and stored first (UPD).  All loads are issued before any
+// threshold applied and stored first (UPD; Re(rhat) from a.grad).  All loads are issued before any
 // value is used (clamped to column 0 where the input is padding, the value
 // then dropped): a load under a per-lane condition would be compiled into a
 // branch whose merge waits for it before the next load is issued.
 template <int NR, bool CPLX, bool UPD>
 __device__ __forceinline__ void row_fwd_inputs(float2 (&v)[16], const RowFwdArgs& a, int t, int s, int X,
                                                bool valid, int h1) {
-  float cj[8], mj[8];
+  float cj[8], mj[8], gj[8];
   float2 rj[8];
   bool ok[8];
   int Yj[8];
@@ -69,14 +69,16 @@ __device__ __forceinline__ void row_fwd_inputs(float2 (&v)[16], const RowFwdArgs
     Yj[j] = ok[j] ? Y : 0;
     cj[j] = a.ck[(size_t)X * a.N1 + Yj[j]];
     if constexpr (!CPLX) mj[j] = a.m[row0 + Yj[j]];
-    if constexpr (CPLX || UPD) rj[j] = a.rhat[row0 + Yj[j]];
+    if constexpr (CPLX) rj[j] = a.rhat[row0 + Yj[j]];
+    if constexpr (UPD) gj[j] = a.grad[row0 + Yj[j]];
   }
   const double al = UPD ? a.alpha[s] : 0.0;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     asm volatile("" : "+v"(cj[j]));
     if constexpr (!CPLX) asm volatile("" : "+v"(mj[j]));
-    if constexpr (CPLX || UPD) asm volatile("" : "+v"(rj[j].x), "+v"(rj[j].y));
+    if constexpr (CPLX) asm volatile("" : "+v"(rj[j].x), "+v"(rj[j].y));
+    if constexpr (UPD) asm volatile("" : "+v"(gj[j]));
   }
 #pragma unroll
   for (int r = 0; r < 16; r++) v[r] = make_float2(0.f, 0.f);
@@ -88,7 +90,7 @@ __device__ __forceinline__ void row_fwd_inputs(float2 (&v)[16], const RowFwdArgs
       float mv = mj[j];
       if constexpr (UPD) {
         // m - al*Re(rhat) in double, stored as float, then the threshold
-        float nv = (float)__fma_rn(-al, (double)rj[j].x, (double)mv);
+        float nv = (float)__fma_rn(-al, (double)gj[j], (double)mv);
         if (a.update == 2) {
           if (a.data_type == 1) { if (nv > a.thr) nv = a.thr; }
           else { if (nv < a.thr) nv = a.thr; }
@@ -129,8 +131,10 @@ __device__ __forceinline__ void row_fwd_body(const RowFwdArgs& a, int bid) {
   fft_regs_half<NR, RowLay, -1>(v, lds + team * kRowLds<NR>, t, Tw);
   if (valid) {
     float2* o = a.out + (size_t)row * NR;
+    const unsigned zw = (unsigned)(a.zhi - a.zlo);
 #pragma unroll
-    for (int i = 0; i < 16; i++) o[out_index<NR>(t, i)] = v[i];
+    for (int i = 0; i < 16; i++)
+      if ((unsigned)(out_index<NR>(t, i) - a.zlo) >= zw) o[out_index<NR>(t, i)] = v[i];
   }
 }
 
@@ -196,7 +200,7 @@ __device__ __forceinline__ void row_adj_outputs(const float2 (&v)[16], float2 (&
       a.img[row0 + Yj[i]] = f;
     } else {
       const float2 r = make_float2(f.x - wj[i].x, f.y - wj[i].y);
-      a.rhat[row0 + Yj[i]] = r;
+      a.grad[row0 + Yj[i]] = r.x;   // the update reads only Re(rhat)
       acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
       if constexpr (FWD) {
         constexpr int RL = last_radix<NR>();
@@ -220,11 +224,21 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
   FftTwiddles Tw;
   load_twiddles<NR, 1>(Tw, t, a.tw);
   // the row (row 0 for a team past the last row, its results dropped): all
-  // 16 loads unconditional, so that they are in flight together
+  // 16 loads unconditional, so that they are in flight together.  Buffer
+  // loads over the row: a column of the zero band gets an offset past the
+  // row, which the load returns as 0 without reading memory.
   float2 v[16];
-  const float2* src = a.in + (size_t)(valid ? row : 0) * NR;
+  {
+    const auto rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(a.in + (size_t)(valid ? row : 0) * NR),
+                                                      (short)0, NR * (int)sizeof(float2), 0x00020000);
+    const unsigned zw = (unsigned)(a.zhi - a.zlo);
 #pragma unroll
-  for (int r = 0; r < 16; r++) v[r] = src[in_index<NR>(t, r)];
+    for (int r = 0; r < 16; r++) {
+      const int y = in_index<NR>(t, r);
+      const unsigned off = (unsigned)(y - a.zlo) >= zw ? (unsigned)y * 8u : 0x80000000u;
+      v[r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    }
+  }
   fft_regs<NR, RowLay, 1>(v, lds + team * kRowLds<NR>, t, Tw);
   const int h1 = a.N1 >> 1;
   double acc = 0.0;
@@ -248,8 +262,10 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
     fft_regs_half<NR, RowLay, -1>(u, lds + team * kRowLds<NR>, t, Tw);
     if (valid) {
       float2* o = a.fwd_out + (size_t)row * NR;
+      const unsigned zw = (unsigned)(a.zhi - a.zlo);
 #pragma unroll
-      for (int i = 0; i < 16; i++) o[out_index<NR>(t, i)] = u[i];
+      for (int i = 0; i < 16; i++)
+        if ((unsigned)(out_index<NR>(t, i) - a.zlo) >= zw) o[out_index<NR>(t, i)] = u[i];
     }
   }
 }
@@ -472,8 +488,10 @@ __device__ __forceinline__ void col_adj_body(const ColArgs& a, int bid) {
   const int s = bid % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;
-  // a tile no node reaches: its spread grid and so its transform are zero
+  // a tile no node reaches: its spread grid and so its transform are zero;
+  // inside the zero band the row kernel takes the zeros without loading them
   if (a.ent_off[blk] == a.ent_off[blk + 1]) {
+    if (y0 >= a.zlo && y0 < a.zhi) return;
     for (int i = tid; i < a.N0 * C; i += NT)
       a.out[((size_t)s * a.N0 + i / C) * a.n1 + y0 + i % C] = make_float2(0.f, 0.f);
     return;
@@ -684,10 +702,18 @@ __device__ __forceinline__ void combine_body(const CombineArgs& a, int s) {
   } else if (a.mode == 2) {
     if (tid == 0) a.bnorm[s] = tot;
   } else {
-    // rtr = sum of the row partials in row order
+    // rtr = sum of the row partials in row order, summed from LDS (red is
+    // free after block_sum_1024's last barrier): the block loads them in one
+    // round trip, where a loop of global loads on one thread waits for each
     double rtr = 0.0;
+    for (int X0 = 0; X0 < a.N0; X0 += 1024) {
+      if (X0 + tid < a.N0) red[tid] = a.rowpart[(size_t)s * a.N0 + X0 + tid];
+      __syncthreads();
+      if (tid == 0)
+        for (int X = 0; X < min(1024, a.N0 - X0); X++) rtr += red[X];
+      __syncthreads();
+    }
     if (tid == 0) {
-      for (int X = 0; X < a.N0; X++) rtr += a.rowpart[(size_t)s * a.N0 + X];
       const double al = tot != 0.0 ? rtr / tot : 0.0;
       a.alpha[s] = al;
       if (a.alpha_trace) a.alpha_trace[(size_t)s * a.alpha_stride + a.alpha_index] = al;

@@ -56,6 +56,26 @@ struct Bins {
   std::vector<float> fpx, fpy;
 };
 
+// The grid columns [zlo, zhi) (FFT order, a multiple of C at both ends) of
+// the run of column tiles without bin entries around column n1/2.  Every
+// tilt line's nodes reach ky in [-k, k], so the tiles the nodes reach form a
+// band around column 0 and the free ones a run around n1/2.  Spread grids
+// are zero there and no interpolation reads a forward grid there, so the
+// row kernels neither load nor store those columns and the spread kernel
+// does not write them.  Empty (0, 0) when the tile holding column n1/2 is
+// reached.
+inline void zero_band(const Bins& b, int* zlo, int* zhi) {
+  *zlo = *zhi = 0;
+  const int mid = b.nblk / 2;
+  auto empty = [&](int t) { return b.off[t] == b.off[t + 1]; };
+  if (b.nblk < 2 || !empty(mid)) return;
+  int lo = mid, hi = mid + 1;
+  while (lo > 0 && empty(lo - 1)) lo--;
+  while (hi < b.nblk && empty(hi)) hi++;
+  *zlo = lo * b.C;
+  *zhi = hi * b.C;
+}
+
 // Fills g; returns false (with a message) on an invalid geometry.
 bool build_geometry(Geometry& g, int N0, int N1, int nz, const float* tilt_deg, std::string* err, bool cv = false,
                     int window = ICON_WINDOW_GAUSS);
@@ -121,7 +141,7 @@ struct icon_plan {
   float2* d_fw = nullptr;           // [S][E] weighted node value of every bin entry
   float2* d_part = nullptr;         // [S][M][kMaxSlot] interpolation partials
   float2* d_ahwb = nullptr;         // [S][N0][N1]
-  float2* d_rhat = nullptr;         // [S][N0][N1]
+  float* d_grad = nullptr;          // [S][N0][N1] Re(rhat), the gradient the update takes
   float2* d_tmp = nullptr;          // [S][N0][n1] row-transform intermediate
   float2* d_frep = nullptr;         // [S][N0] forward values at the omit tilt
   double* d_rowpart = nullptr;      // [S][N0] per-row partial sums

@@ -345,7 +345,8 @@ class Plan:
                                       _ptr(d_resid), _ptr(d_alpha), ctypes.byref(opts), _ptr(stream)),
                "icon_reconstruct")
 
-    KERNELS = {"row_fwd": 0, "col_fwd": 1, "combine": 2, "col_adj": 3, "row_adj": 4, "sino": 5}
+    KERNELS = {"row_fwd": 0, "col_fwd": 1, "combine": 2, "col_adj": 3, "row_adj": 4, "sino": 5,
+               "combine_alpha": 6}
 
     def time_kernel(self, name, S, reps=20):
         """Average launch time (ms) of one iteration kernel over S slices (HIP events on the plan stream)."""

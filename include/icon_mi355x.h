@@ -254,7 +254,8 @@ typedef enum {
   ICON_K_ROW_ADJ = 4,   /* y-IFFT + crop + deapodise + gradient + |r|^2 (FP64)
                            + deapodise, pad and y-FFT of the gradient (the row
                            pass of A rhat, fused)                              */
-  ICON_K_SINO = 5       /* 1D transform of the projection rows                 */
+  ICON_K_SINO = 5,      /* 1D transform of the projection rows                 */
+  ICON_K_COMBINE_ALPHA = 6  /* node sums of A rhat, sum w|A rhat|^2 (FP64), alpha */
 } icon_kernel_id;
 
 /* Times `reps` launches of one kernel of the iteration over S slices, each
