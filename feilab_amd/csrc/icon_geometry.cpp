@@ -283,6 +283,11 @@ void build_bins(const Geometry& g, int C, Bins& b) {
     }
   }
   for (int i = 0; i < b.nblk; i++) b.off[i + 1] += b.off[i];
+  b.order.resize(b.nblk);
+  for (int i = 0; i < b.nblk; i++) b.order[i] = i;
+  std::stable_sort(b.order.begin(), b.order.end(), [&](int u, int v) {
+    return b.off[u + 1] - b.off[u] > b.off[v + 1] - b.off[v];
+  });
   b.ent.resize(b.off[b.nblk]);
   std::vector<int> fill(b.off.begin(), b.off.end() - 1);
   for (int j = 0; j < g.M; j++) {

@@ -47,6 +47,7 @@ struct ColArgs {
   float2* out;             // adj: tmp [S][N0][n1]
   const float2* tw;
   const int* ent_off;
+  const int* tile_order;   // adj: [nblk] tile of each block group (blocks bid / S), largest first
   const int4* ent;
   const int* xoff;         // adj: [nblk][n0+1] row offsets into ent
   const float* phix;

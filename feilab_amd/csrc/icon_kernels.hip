@@ -348,7 +348,9 @@ __device__ __forceinline__ void col_fwd_body(const ColArgs& a, int bid) {
   // tile of different slices and share its bin records in L2 (an order that
   // put the two tiles sharing each 128-byte grid line on one XCD back to back
   // halved the L2 misses but ran 3 % slower: the repeated lines come from
-  // the memory-side cache)
+  // the memory-side cache).  (The spread's order by entry count measured no
+  // faster here: an interpolation block's time varies far less with its
+  // tile's entries.)
   const int blk = bid / a.S;
   const int s = bid % a.S;
   const int y0 = blk * C;
@@ -483,8 +485,10 @@ __device__ __forceinline__ void col_adj_body(const ColArgs& a, int bid) {
   constexpr int CAP = col_lds<NC, C>() * 2 / REC;  // records staged at once in the tile's LDS
   extern __shared__ __align__(16) float2 tile[];   // col_lds<NC, C>()
   // tile-major order: blocks in flight together work on the same column
-  // tile of different slices and share its bin records in L2
-  const int blk = bid / a.S;
+  // tile of different slices and share its bin records in L2; the tiles by
+  // entry count, largest first, so that the dense tiles' long blocks start
+  // first and short ones fill the end of the launch
+  const int blk = a.tile_order[bid / a.S];
   const int s = bid % a.S;
   const int y0 = blk * C;
   const int tid = threadIdx.x;

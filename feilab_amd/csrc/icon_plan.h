@@ -54,6 +54,12 @@ struct Bins {
   // (records interleaved by field per tile as above).
   std::vector<int4> fent;
   std::vector<float> fpx, fpy;
+  // Launch order of the spread's tiles: by entry count, largest first (ties
+  // by index).  A spread block costs about its tile's entries, and the few
+  // dense tiles around ky = 0 (up to 10x the median block) would otherwise
+  // run at the start and the very end of the tile-major order, the last of
+  // them alone on a nearly idle device.
+  std::vector<int> order;
 };
 
 // The grid columns [zlo, zhi) (FFT order, a multiple of C at both ends) of
@@ -123,6 +129,7 @@ struct icon_plan {
   float* d_phiy = nullptr;
   int* d_ix0 = nullptr;
   int* d_ent_off = nullptr;
+  int* d_tile_order = nullptr;      // Bins::order
   int4* d_ent = nullptr;
   int* d_xoff = nullptr;
   float* d_ent_px = nullptr;
