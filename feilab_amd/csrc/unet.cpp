@@ -81,6 +81,7 @@ struct icon_unet {
   float* adam_v = nullptr;
   long adam_t = 0;
   double* d_loss = nullptr;
+  float* gpart = nullptr;    // training: partial sums of the weight, bias and head gradients
   // activations (bf16)
   T16 x8, a1, cat1, p1, a2, cat2, p2, a3, cat3, p3, ab, bb, d3a, d3b, d2a, d2b, d1a, d1b;
   float* out = nullptr;      // [maxB][H][W] network output of the warm start
@@ -178,16 +179,19 @@ int conv(icon_unet* n, const Conv& c, const TX* x, int ldx, int cx0, TY* y, int 
   return 0;
 }
 
+// pixel splits of a weight gradient over P pixels
+int wgrad_splits(long P) { return (int)std::max<long>(1, std::min<long>(256, P / 2048)); }
+
 int wgrad(icon_unet* n, const Conv& c, const uint16_t* x, int ldx, int cx0, const float* dy, int lddy, int cdy0,
           int B, int H, int W, hipStream_t st) {
   WgradArgs a{};
   a.x = x; a.ldx = ldx; a.cx0 = cx0; a.dy = dy; a.lddy = lddy; a.cdy0 = cdy0;
   a.dw = n->grads + c.off_w;
   a.N = B; a.H = H; a.W = W; a.Cin = c.cin; a.Cout = c.cout; a.taps = c.taps;
+  a.part = n->gpart;
   const long P = (long)B * H * W;
-  const int splits = (int)std::max<long>(1, std::min<long>(256, P / 2048));
-  UCHK(launch_wgrad(a, splits, st));
-  UCHK(launch_bias_grad(dy, lddy, cdy0, P, c.cout, n->grads + c.off_b, st));
+  UCHK(launch_wgrad(a, wgrad_splits(P), st));
+  UCHK(launch_bias_grad(dy, lddy, cdy0, P, c.cout, n->grads + c.off_b, n->gpart, st));
   return 0;
 }
 
@@ -266,7 +270,7 @@ int backward(icon_unet* n, int B, const float* d_dout, hipStream_t st) {
 #define R(x) do { if ((rc = (x))) return rc; } while (0)
 #define HIPR(x) do { if ((x) != hipSuccess) return ufail(2, "%s failed", #x); } while (0)
   HIPR(launch_head_bwd(n->d1b.p, 32, n->params + n->head_w, 32, P, d_dout, n->g_d1b.p, 32, n->grads + n->head_w,
-                       n->grads + n->head_b, st));
+                       n->grads + n->head_b, n->gpart, st));
   // d1b, d1a
   R(wgrad(n, cv[D1B], n->d1a.p, 32, 0, n->g_d1b.p, 32, 0, B, H, W, st));
   R(conv(n, cv[D1B], n->g_d1b.p, 32, 0, n->g_d1a.p, 32, 0, B, H, W, false, n->d1a.p, 32, 0, true, false, st));
@@ -388,6 +392,16 @@ int icon_unet_create(icon_unet** out, int device, int max_batch, int H, int W, i
       (rc = t16(n, n->d2b, P2, 64)) || (rc = t16(n, n->d1a, P, 32)) || (rc = t16(n, n->d1b, P, 32)))
     return bail(rc);
   if (n->train) {
+    // gradient partial sums: the largest of a weight gradient's splits x
+    // weights (at the level of each convolution), the bias and head blocks
+    long part = (long)kHeadGradParts * 33;
+    const long level_px[NCONV] = {P, P, P2, P2, P4, P4, P8, P8, P4, P4, P2, P2, P, P};
+    for (int i = 0; i < NCONV; i++) {
+      const Conv& c = n->convs[i];
+      part = std::max(part, (long)wgrad_splits(level_px[i]) * c.taps * c.cout * c.cin);
+      part = std::max(part, (long)kBiasGradParts * c.cout);
+    }
+    if ((rc = alloc_bytes(n, (void**)&n->gpart, sizeof(float) * part))) return bail(rc);
     if ((rc = alloc_bytes(n, (void**)&n->grads, sizeof(float) * off)) ||
         (rc = alloc_bytes(n, (void**)&n->adam_m, sizeof(float) * off)) ||
         (rc = alloc_bytes(n, (void**)&n->adam_v, sizeof(float) * off)))

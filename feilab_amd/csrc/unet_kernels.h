@@ -59,6 +59,7 @@ struct WgradArgs {
   int lddy, cdy0;
   float* dw;
   int N, H, W, Cin, Cout, taps;
+  float* part;   // [splits][taps][Cout][Cin] scratch: each pixel split's sums, added to dw in split order
 };
 
 // 2x2 average pool (fwd: in H x W -> out H/2 x W/2) or nearest 2x upsample
@@ -84,7 +85,14 @@ struct PoolGradArgs {
 template <typename TX, typename TY>
 hipError_t launch_conv(const ConvArgs<TX, TY>& a, hipStream_t st);
 hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st);
-hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db, hipStream_t st);
+// Weight and bias gradients are reduced without atomics: every block writes
+// its partial sums to a scratch buffer and launch_sum_parts adds them to the
+// gradient in a fixed order, so that training is reproducible run to run.
+hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db, float* part, hipStream_t st);
+constexpr int kBiasGradParts = 256;   // partial sums per channel (launch_bias_grad's part: [256][C])
+constexpr int kHeadGradParts = 256;   // blocks of launch_head_bwd (its part: [256][C + 1])
+// out[i] += part[0][i] + part[1][i] + ... + part[nparts - 1][i] (in that order)
+hipError_t launch_sum_parts(const float* part, int nparts, long n, float* out, hipStream_t st);
 hipError_t launch_pool_bwd(const PoolGradArgs& a, hipStream_t st);
 hipError_t launch_up_fwd(const PoolArgs& a, hipStream_t st);
 hipError_t launch_up_bwd(const PoolGradArgs& a, hipStream_t st);
@@ -93,7 +101,7 @@ hipError_t launch_relu_mask(float* g, int ldg, int cg0, const uint16_t* act, int
 hipError_t launch_head_fwd(const uint16_t* x, int ldx, const float* w, const float* b, int C, long P, float* out,
                            hipStream_t st);
 hipError_t launch_head_bwd(const uint16_t* x, int ldx, const float* w, int C, long P, const float* dout, float* dx,
-                           int lddx, float* dw, float* db, hipStream_t st);
+                           int lddx, float* dw, float* db, float* part, hipStream_t st);
 hipError_t launch_flip_weights(const uint16_t* w, uint16_t* wt, int taps, int Cout, int Cin, hipStream_t st);
 hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long n, float lr, float b1,
                        float b2, float eps, float c1, float c2, float gscale, hipStream_t st);

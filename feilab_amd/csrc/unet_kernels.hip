@@ -598,8 +598,8 @@ __global__ __launch_bounds__(256, COB == 32 && R == 4 ? 3 : 2) void k_conv(ConvA
 // ------------------------------------------------------------ weight grad
 // dW[tap][co][ci] += sum_p dY[p][co] * X[p + off(tap)][ci] over a pixel chunk.
 // Block 256 threads: output tile 64 co x 64 ci for one tap; K = pixels in
-// steps of 32 staged (transposed) in LDS; split-K over blockIdx.z with
-// float atomics into dW.
+// steps of 32 staged (transposed) in LDS; split-K over blockIdx.z, each split
+// writing its sums to part[z] (added to dW in split order afterwards).
 __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
   __shared__ uint16_t sdy[64][40];   // [co][pixel], padded rows
   __shared__ uint16_t sx[64][40];    // [ci][pixel]
@@ -682,14 +682,15 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
       for (int r = 0; r < 4; r++) {
         const int co = co_t * 64 + co_w + 16 * i + 4 * g + r;
         const int ci = ci_t * 64 + ci_w + 16 * j + r16;
-        if (co < a.Cout && ci < a.Cin) atomicAdd(&a.dw[((long)tap * a.Cout + co) * a.Cin + ci], acc[i][j][r]);
+        if (co < a.Cout && ci < a.Cin)
+          a.part[(((long)blockIdx.z * a.taps + tap) * a.Cout + co) * a.Cin + ci] = acc[i][j][r];
       }
 }
 
 // ------------------------------------------------------------ small ops
 
-// db[c] += sum_p dY[p][c]
-__global__ __launch_bounds__(256) void k_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db) {
+// parts[blockIdx.y][c] = this block's share of sum_p dY[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(const float* dy, int ld, int c0, long P, int C, float* parts) {
   const int c = blockIdx.x * 64 + (threadIdx.x & 63);
   const int part = threadIdx.x >> 6;
   __shared__ float red[4][64];
@@ -698,7 +699,8 @@ __global__ __launch_bounds__(256) void k_bias_grad(const float* dy, int ld, int 
     for (long p = blockIdx.y * 4 + part; p < P; p += 4L * gridDim.y) s += dy[p * ld + c0 + c];
   red[part][threadIdx.x & 63] = s;
   __syncthreads();
-  if (part == 0 && c < C) atomicAdd(&db[c], red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
+  if (part == 0 && c < C)
+    parts[(long)blockIdx.y * C + c] = (red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // 2x2 average pool (bf16 NHWC, channel sub-ranges); 8 channels per thread
@@ -769,9 +771,10 @@ __global__ void k_head_fwd(const uint16_t* x, int ldx, const float* w, const flo
   }
 }
 
-// head backward: dX[p][c] = dout[p] w[c] (masked by ReLU of x); dw, db reduced
+// head backward: dX[p][c] = dout[p] w[c] (masked by ReLU of x); each block's
+// sums of dw and db to part[block] (added in block order afterwards)
 __global__ __launch_bounds__(256) void k_head_bwd(const uint16_t* x, int ldx, const float* w, int C, long P,
-                                                 const float* dout, float* dx, int lddx, float* dw, float* db) {
+                                                 const float* dout, float* dx, int lddx, float* part) {
   __shared__ float red[256];
   float gw[64];
   for (int c = 0; c < C; c++) gw[c] = 0.f;
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(256) void k_head_bwd(const uint16_t* x, int ldx, co
       if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
       __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(c < C ? &dw[c] : db, red[0]);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * (C + 1) + c] = red[0];   // [block][dw..., db]
     __syncthreads();
   }
 }
@@ -958,13 +961,32 @@ template hipError_t launch_conv<uint16_t, uint16_t>(const ConvArgs<uint16_t, uin
 template hipError_t launch_conv<float, float>(const ConvArgs<float, float>&, hipStream_t);
 
 hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st) {
+  if (!a.part || splits < 1) return hipErrorInvalidValue;
   const int nco = (a.Cout + 63) / 64, nci = (a.Cin + 63) / 64;
   hipLaunchKernelGGL(k_conv_wgrad, dim3(a.taps * nco * nci, 1, splits), dim3(256), 0, st, a);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_sum_parts(a.part, splits, (long)a.taps * a.Cout * a.Cin, a.dw, st);
 }
 
-hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db, hipStream_t st) {
-  hipLaunchKernelGGL(k_bias_grad, dim3((C + 63) / 64, 256), dim3(256), 0, st, dy, ld, c0, P, C, db);
+hipError_t launch_bias_grad(const float* dy, int ld, int c0, long P, int C, float* db, float* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_bias_grad, dim3((C + 63) / 64, kBiasGradParts), dim3(256), 0, st, dy, ld, c0, P, C, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_sum_parts(part, kBiasGradParts, C, db, st);
+}
+
+__global__ void k_sum_parts(const float* __restrict__ part, int nparts, long n, float* out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float s = part[i];
+    for (int z = 1; z < nparts; z++) s += part[(long)z * n + i];
+    out[i] += s;
+  }
+}
+
+hipError_t launch_sum_parts(const float* part, int nparts, long n, float* out, hipStream_t st) {
+  if (n <= 0 || nparts <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sum_parts, dim3(blocks_for(n)), dim3(256), 0, st, part, nparts, n, out);
   return hipGetLastError();
 }
 
@@ -991,9 +1013,13 @@ hipError_t launch_head_fwd(const uint16_t* x, int ldx, const float* w, const flo
   return hipGetLastError();
 }
 hipError_t launch_head_bwd(const uint16_t* x, int ldx, const float* w, int C, long P, const float* dout, float* dx,
-                           int lddx, float* dw, float* db, hipStream_t st) {
-  hipLaunchKernelGGL(k_head_bwd, dim3(256), dim3(256), 0, st, x, ldx, w, C, P, dout, dx, lddx, dw, db);
-  return hipGetLastError();
+                           int lddx, float* dw, float* db, float* part, hipStream_t st) {
+  // dw and db are adjacent in the parameter vector (db = dw + C)
+  if (db != dw + C) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_head_bwd, dim3(kHeadGradParts), dim3(256), 0, st, x, ldx, w, C, P, dout, dx, lddx, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_sum_parts(part, kHeadGradParts, C + 1, dw, st);
 }
 hipError_t launch_flip_weights(const uint16_t* w, uint16_t* wt, int taps, int Cout, int Cin, hipStream_t st) {
   hipLaunchKernelGGL(k_flip_weights, dim3(blocks_for((long)taps * Cout * Cin)), dim3(256), 0, st, w, wt, taps,
