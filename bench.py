@@ -15,11 +15,14 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   phantoms; one step = 10 plain iterations, the U-Net warm start, k descent
   iterations with residuals and reprojection, the residuals read back, and
   plain ICON re-solving every slice whose R misses its target (the per-slice
-  fallback, timed as part of the step).  k is calibrated once on a held-out
-  phantom of the training distribution, never on the evaluated series: the
-  smallest k at which every calibration slice reaches 99 % of its plain-ICON
-  residual (the margin keeps fallbacks on the evaluated series rare) with an
-  FRC against the phantom no more than 0.005 below plain ICON's.
+  fallback, timed as part of the step).  k, with an output gain g of the
+  warm start (the start is g (m + s U(m / s))), is calibrated once on a
+  held-out phantom of the training distribution, never on the evaluated
+  series: the smallest k at which, for some g of --gains, every calibration
+  slice reaches 99 % of its plain-ICON residual (the margin keeps fallbacks
+  on the evaluated series rare) with an FRC against the phantom no more than
+  0.005 below plain ICON's; of those g, the one with the lowest worst-slice
+  residual.
 * Quality (not timed): per slice, relative L2 and FRC of the learned result
   against plain ICON's and against the phantom, and the same for the 10 + k
   iterations without the network.
@@ -305,7 +308,7 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
     solver = warmstart.LearnedSolver(plan, params, S, start=args.start, schedule=sched, batch=args.unet_batch,
                                      overlap=bool(args.overlap))
 
-    # continuation length from a held-out phantom of the training distribution
+    # continuation length (and output gain) from a held-out phantom of the training distribution
     Sc = args.calib_slices
     cal = phantom.config2(nslices=Sc, seed=3001)
     truth_c = phantom.render_slices_3d(cal["blobs"], N, N, np.arange(Sc))
@@ -316,21 +319,33 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
     L.icon_device_synchronize()
     R_c = d_rc.numpy().reshape(-1)[:Sc * (iters + 1)].reshape(Sc, iters + 1)[:, -1]
     q_plain_c = band_mean(F.frc_slices(d_mc.numpy(), truth_c), band)
-    k_star, scan = None, []
+    # the smallest continuation k at which, for some output gain g of the
+    # network's start (icon_unet_set_output_gain), every calibration slice
+    # meets the residual and FRC targets; of the gains that do, the one with
+    # the lowest worst-slice residual
+    gains = [float(v) for v in args.gains.split(",")]
+    k_star, g_star, scan = None, None, []
     for k in range(1, args.max_cont + 1):
-        solver.run(Sc, d_c, d_mc, k, d_resid=d_rc)
-        L.icon_device_synchronize()
-        Rk = d_rc.numpy().reshape(-1)[:Sc * (k + 1)].reshape(Sc, k + 1)[:, -1]
-        qk = band_mean(F.frc_slices(d_mc.numpy(), truth_c), band)
-        row = dict(k=k, resid_max_over_target=round(float((Rk / R_c).max()), 4),
-                   frc_truth_drop_max=round(float((q_plain_c - qk).max()), 4),
-                   frc_truth_mean=round(float(qk.mean()), 4))
-        scan.append(row)
-        log("calibration k=%d:" % k, json.dumps(row))
-        if (Rk <= (1.0 - args.calib_margin) * R_c).all() and (qk >= q_plain_c - args.frc_tol).all():
-            k_star = k
+        passing = []
+        for g in gains:
+            solver.set_gain(g)
+            solver.run(Sc, d_c, d_mc, k, d_resid=d_rc)
+            L.icon_device_synchronize()
+            Rk = d_rc.numpy().reshape(-1)[:Sc * (k + 1)].reshape(Sc, k + 1)[:, -1]
+            qk = band_mean(F.frc_slices(d_mc.numpy(), truth_c), band)
+            row = dict(k=k, gain=g, resid_max_over_target=round(float((Rk / R_c).max()), 4),
+                       frc_truth_drop_max=round(float((q_plain_c - qk).max()), 4),
+                       frc_truth_mean=round(float(qk.mean()), 4))
+            scan.append(row)
+            log("calibration k=%d gain=%.2f:" % (k, g), json.dumps(row))
+            if (Rk <= (1.0 - args.calib_margin) * R_c).all() and (qk >= q_plain_c - args.frc_tol).all():
+                passing.append((float((Rk / R_c).max()), g))
+        if passing:
+            k_star, g_star = k, min(passing)[1]
             break
-    calib = dict(phantom_seed=3001, slices=Sc, resid_margin=args.calib_margin,
+    if g_star is not None:
+        solver.set_gain(g_star)
+    calib = dict(phantom_seed=3001, slices=Sc, resid_margin=args.calib_margin, gains=gains,
                  frc_truth_plain_mean=round(float(q_plain_c.mean()), 4), scan=scan)
     for a in (d_c, d_mc, d_rc):
         a.free()
@@ -389,7 +404,7 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
                            no_network_frc=st(f_np)),
         frc_band=list(band),
         note="per slice; FRC = mean over rings %d..%d of calFRC's ring correlation of the two slices" % band)
-    learned = dict(start_iterations=args.start, continuation_iterations=k_star,
+    learned = dict(start_iterations=args.start, continuation_iterations=k_star, output_gain=g_star,
                    slices_per_s=round(value, 3), speedup_vs_plain=round(value / plain_value, 3),
                    resid_mean=float(R_l.mean()), resid_max_over_target=round(float((R_l / R_plain).max()), 5),
                    fallback_slices_per_step=fallbacks[-1], fallback_slices_all_steps=fallbacks,
@@ -505,6 +520,8 @@ def main():
     ap.add_argument("--calib-margin", type=float, default=0.01,
                     help="calibration target: R <= (1 - margin) x plain ICON's R on every calibration slice")
     ap.add_argument("--max-cont", type=int, default=40, help="continuation iterations tried")
+    ap.add_argument("--gains", default="1,0.95,0.9,0.85,0.8",
+                    help="output gains of the warm start tried with each continuation length")
     ap.add_argument("--frc-tol", type=float, default=0.005,
                     help="allowed drop of the FRC against the phantom below plain ICON's")
     ap.add_argument("--no-learned", action="store_true")
@@ -662,8 +679,9 @@ def main():
         log("cpu baseline:", json.dumps(cpu))
     workload = ("learned-start ICON vs plain ICON: %d slices of 512x512, 41 tilts +-60 deg step 3, SNR 0.05; "
                 "target = per-slice data residual of plain ICON (%d iterations %d,%d,%d, m >= 0), the continuation "
-                "length calibrated on a held-out training-distribution phantom where every slice reaches 99%% of its "
-                "plain-ICON residual and keeps its FRC against the phantom within %.3f of plain ICON's; U-Net "
+                "length and the warm start's output gain calibrated on a held-out training-distribution phantom "
+                "where every slice reaches 99%% of its plain-ICON residual and keeps its FRC against the phantom "
+                "within %.3f of plain ICON's; U-Net "
                 "trained on %d (%d, %d)-iteration pairs; "
                 "per-slice fallback to plain ICON timed in the step; sinograms resident in HBM, solves replayed as "
                 "HIP graphs" % (S, iters, pre, icon_it, post, args.frc_tol, args.train_pairs, args.start, iters))

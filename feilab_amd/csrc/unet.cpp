@@ -86,6 +86,7 @@ struct icon_unet {
   T16 x8, a1, cat1, p1, a2, cat2, p2, a3, cat3, p3, ab, bb, d3a, d3b, d2a, d2b, d1a, d1b;
   float* out = nullptr;      // [maxB][H][W] network output of the warm start
   float* scale = nullptr;    // [maxB] per-image input scale of the warm start
+  float out_gain = 1.f;      // factor on the warm start's output (icon_unet_set_output_gain)
   double* rms_part = nullptr;  // [maxB][image_rms_parts()] partial sums of the scale
   // gradients (fp32)
   T32 g_d1b, g_d1a, g_cat1, g_d2b, g_d2a, g_cat2, g_d3b, g_d3a, g_cat3, g_bb, g_ab, g_p3, g_a3, g_p2, g_a2, g_p1,
@@ -482,8 +483,14 @@ int icon_unet_warm_start(icon_unet* n, int S, const float* d_m, float* d_out, vo
     UCHK(launch_image_rms(m, P, B, n->rms_part, n->scale, st));
     int rc = forward(n, B, m, n->out, st, n->scale);
     if (rc) return rc;
-    UCHK(launch_residual_add(m, n->out, n->scale, P, B, d_out + (long)s0 * P, st));
+    UCHK(launch_residual_add(m, n->out, n->scale, P, B, n->out_gain, d_out + (long)s0 * P, st));
   }
+  return 0;
+}
+
+int icon_unet_set_output_gain(icon_unet* n, float gain) {
+  if (!n || !(gain > 0.f) || !(gain < 16.f)) return ufail(1, "icon_unet_set_output_gain: need 0 < gain < 16");
+  n->out_gain = gain;
   return 0;
 }
 

@@ -112,8 +112,9 @@ hipError_t launch_mse(const float* pred, const float* target, long P, float* dpr
 hipError_t launch_image_rms(const float* in, long P, int B, double* part, float* scale, hipStream_t st);
 int image_rms_parts();
 hipError_t launch_to_bf16_scaled(const float* in, const float* scale, long P, int B, uint16_t* out, hipStream_t st);
-hipError_t launch_residual_add(const float* m, const float* y, const float* scale, long P, int B, float* out,
-                               hipStream_t st);
+// out = gain * (m + scale[b] * y) per image b (gain 1: m + scale[b] * y exactly)
+hipError_t launch_residual_add(const float* m, const float* y, const float* scale, long P, int B, float gain,
+                               float* out, hipStream_t st);
 hipError_t launch_zero32(float* p, long n, hipStream_t st);
 
 }  // namespace unet

@@ -28,6 +28,7 @@ def _lib():
             "icon_unet_set_params": [vp, vp, ctypes.c_int64],
             "icon_unet_forward": [vp, ci, vp, vp, vp],
             "icon_unet_warm_start": [vp, ci, vp, vp, vp],
+            "icon_unet_set_output_gain": [vp, ctypes.c_float],
             "icon_unet_gradients": [vp, ci, vp, vp, ctypes.POINTER(ctypes.c_double), vp],
             "icon_unet_get_gradients": [vp, vp, ctypes.c_int64],
             "icon_unet_train_step": [vp, ci, vp, vp, ctypes.c_float, ctypes.POINTER(ctypes.c_double), vp],
@@ -101,8 +102,13 @@ class UNet:
         self.forward_dev(B, d_in, d_out)
         return d_out.numpy()
 
+    def set_output_gain(self, gain):
+        """Factor g on the warm start's result: g * (m + s * U(m / s)) (default 1)."""
+        _check(_lib().icon_unet_set_output_gain(self.h, float(gain)), "set_output_gain")
+
     def warm_start_dev(self, S, d_m, d_out, stream=None):
-        """d_out = m + s * U(m / s) per slice, s = RMS of the slice (device buffers)."""
+        """d_out = m + s * U(m / s) per slice, s = RMS of the slice (device buffers),
+        times the output gain."""
         _check(_lib().icon_unet_warm_start(self.h, S, _p(d_m), _p(d_out), _s(stream)), "warm_start")
 
     def warm_start(self, m):

@@ -179,6 +179,7 @@ class LearnedSolver:
         half = (max_slices + 1) // 2
         self.net = UNet(plan.N0, plan.N1, max_batch=min(batch, half if self.overlap else max_slices), device=device)
         self.net.set_params(params)
+        self.gain = 1.0
         shape = (max_slices, plan.N0, plan.N1)
         self.d_start = DeviceArray(shape, np.float32)
         self.d_init = DeviceArray(shape, np.float32)
@@ -208,6 +209,14 @@ class LearnedSolver:
         plan.reconstruct_dev(S, d_sino, d_m, SolveOptions(cont, 0, 0, self.data_type, self.threshold, 1, 1),
                              d_m_init=d_init if use_net else d_start, d_resid=d_resid, d_reproj=d_reproj,
                              stream=stream)
+
+    def set_gain(self, gain):
+        """Factor on the network's warm start (icon_unet_set_output_gain), for
+        the network(s) of this solver."""
+        self.gain = float(gain)
+        for net in (self.net, getattr(self, "net2", None)):
+            if net is not None:
+                net.set_output_gain(self.gain)
 
     def run(self, S, d_sino, d_m, cont, d_resid=None, d_reproj=None, stream=None, use_net=True):
         """start plain iterations -> U-Net -> `cont` descent iterations into d_m;

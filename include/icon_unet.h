@@ -38,6 +38,12 @@ int icon_unet_forward(icon_unet* net, int B, const float* d_in, float* d_out, vo
  * a converged one.  d_out may equal d_m.  The result seeds icon_reconstruct's
  * d_m_init. */
 int icon_unet_warm_start(icon_unet* net, int S, const float* d_m, float* d_out, void* stream);
+/* Factor g on the warm start's result: d_out = g * (d_m + s * U-Net(d_m / s)).
+ * Default 1 (the result exactly as above).  A learned-start solver chooses it
+ * with the continuation length on held-out data: the descent iterations that
+ * follow reach a given data residual sooner from a start a little below the
+ * network's amplitude. */
+int icon_unet_set_output_gain(icon_unet* net, float gain);
 /* Gradients of loss = mean (U-Net(in) - target)^2 with respect to the
  * parameters (training net); *loss receives the loss. */
 int icon_unet_gradients(icon_unet* net, int B, const float* d_in, const float* d_target, double* loss, void* stream);

@@ -146,6 +146,15 @@ def test_unet_warm_start_is_scaled_residual(icon):
     ref = m + s[:, None, None] * y
     for i in range(5):
         assert rel(out[i], ref[i]) < 2e-3, i   # host and device round m / s to BF16 separately
+    # the output gain multiplies the result (one FP32 product per pixel), and
+    # gain 1 is the plain result again
+    net.set_output_gain(0.85)
+    assert np.array_equal(net.warm_start(m), (np.float32(0.85) * out).astype(np.float32))
+    net.set_output_gain(1.0)
+    assert np.array_equal(net.warm_start(m), out)
+    from feilab_amd.unet import UNetError
+    with pytest.raises(UNetError):
+        net.set_output_gain(0.0)
     net.close()
 
 
