@@ -193,7 +193,7 @@ void fill_args(icon_batch* b, const float* const* sino, float* const* m, float* 
     ra.img = p->d_ahwb; ra.ahwb = p->d_ahwb; ra.grad = p->d_grad; ra.rowpart = p->d_rowpart; ra.fwd_out = p->d_tmp;
     ColArgs& ca = b->h_ca[i];
     ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.E = E; ca.K = g.K;
-    ca.in = p->d_tmp; ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.ent = p->d_ent;
+    ca.in = p->d_tmp; ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.split_min = split_min_entries(p->bins); ca.ent = p->d_ent;
     ca.xoff = p->d_xoff; ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.ent_px = p->d_ent_px;
     ca.ent_py = p->d_ent_py; ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
     ca.part = p->d_part; ca.fw = p->d_fw;

@@ -193,7 +193,7 @@ int op_forward(icon_plan* p, int S, const void* img, int cplx, float2* out, hipS
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
-  ca.in = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.in = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.split_min = split_min_entries(p->bins); ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.part = p->d_part;
   if ((rc = check_launch(launch_col(g.n0, p->bins.C, false, ca, st), "col_fwd"))) return rc;
@@ -210,7 +210,7 @@ int op_adjoint_ent(icon_plan* p, int S, float2* img, hipStream_t st) {
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1;
-  ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.out = p->d_tmp; ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.split_min = split_min_entries(p->bins); ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py;
   ca.phix = p->d_phix; ca.phiy = p->d_phiy; ca.ix0 = p->d_ix0; ca.fw = f;
   zero_band(p->bins, &ca.zlo, &ca.zhi);
@@ -271,7 +271,7 @@ int enqueue_solve(icon_plan* p, int S, const float* sino, float* m, const float*
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
-  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.split_min = split_min_entries(p->bins); ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
   CombineArgs cm{};
@@ -784,7 +784,7 @@ int icon_time_kernel(icon_plan* p, int kernel, int S, int reps, double* avg_ms) 
   ColArgs ca{};
   ca.K = p->g.K;
   ca.S = S; ca.N0 = g.N0; ca.N1 = g.N1; ca.M = g.M; ca.n1 = g.n1; ca.in = p->d_tmp; ca.out = p->d_tmp;
-  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
+  ca.tw = p->d_tw0; ca.ent_off = p->d_ent_off; ca.tile_order = p->d_tile_order; ca.split_min = split_min_entries(p->bins); ca.ent = p->d_ent; ca.xoff = p->d_xoff; ca.E = (int)p->bins.ent.size(); ca.ent_px = p->d_ent_px; ca.ent_py = p->d_ent_py;
   ca.fent = p->d_fent; ca.fent_px = p->d_fent_px; ca.fent_py = p->d_fent_py; ca.phix = p->d_phix; ca.phiy = p->d_phiy;
   ca.ix0 = p->d_ix0; ca.part = p->d_part; ca.fw = p->d_fw;
   CombineArgs cm{};

@@ -62,6 +62,7 @@ struct ColArgs {
   float2* part;            // fwd: [S][M][3]
   const float2* fw;        // adj: [S][E] weighted value per bin entry
   int zlo, zhi;            // adj: tiles inside columns [zlo, zhi) store nothing (their zeros are implied)
+  int split_min;           // adj: tiles with at least this many entries split their rows' entries in two
 };
 
 struct CombineArgs {

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "icon_fft.h"
 #include "icon_kernels.h"
@@ -500,38 +501,42 @@ __device__ __forceinline__ void col_adj_body(const ColArgs& a, int bid) {
       a.out[((size_t)s * a.N0 + i / C) * a.n1 + y0 + i % C] = make_float2(0.f, 0.f);
     return;
   }
-  // Spread as a gather.  Thread tid owns the RL adjacent grid rows
-  // [x0, x0+RL) of the tile (C cells each, in registers).  The entries that
-  // cover them are those with first x-tap z in [x0-13, x0+RL-1] (mod n0): one
-  // or two index ranges of the tile's z-sorted list.  The block stages the
-  // tile's records in LDS (CAP at a time) and every thread adds its whole
-  // ranges, two entries per step, each record's taps shared by its RL rows.
-  // No atomics; every cell sums its entries in list order.
+  // Spread as a gather.  Rows [x, x+RL) of the tile are covered by the
+  // entries with first x-tap z in [x-13, x+RL-1] (mod n0): one or two index
+  // ranges of the tile's z-sorted list (run 0: the entries whose taps wrap
+  // from the top rows, for the first K-1 rows only, else empty; run 1: the
+  // rest).  The block stages the tile's records in LDS (CAP at a time) and
+  // thread tid adds the entries of its rows x0 = RL tid, two per step, each
+  // record's taps shared by its RL rows.  No atomics; every cell sums its
+  // entries in list order.
+  //
+  // Dense tiles (at least a.split_min entries: the tiles around ky = 0,
+  // where every tilt line reaches the rows around kx = 0 and their threads
+  // hold up to 13x the tile's mean entries) split each row pair's entries,
+  // run 0 then run 1, in two halves: thread tid adds the first half of its
+  // own rows' into acc and the second half of its partner's (thread
+  // tid ^ NT/2, rows x0 ^ NC/2, light where tid's are heavy) into accp.
+  // A cell there is (its first half in list order) + (its second half in
+  // list order).
   const int* xo = a.xoff + (size_t)blk * (NC + 1);
   const float2* fw = a.fw + (size_t)s * a.E;
   float* stage = reinterpret_cast<float*>(tile);
-  const int x0 = tid * RL;
-  // range 0: the entries whose taps wrap from the top rows (threads of the
-  // first K-1 rows only; empty, [xo[NC], xo[NC]), for the others); range 1:
-  // the rest.  The four loads are unconditional, so that they are in flight
-  // together.
-  const bool wrap = x0 < K - 1;
-  int lo[2], hi[2];
-  lo[0] = xo[wrap ? NC - (K - 1) + x0 : NC];
-  hi[0] = xo[NC];
-  lo[1] = xo[wrap ? 0 : x0 - (K - 1)];
-  hi[1] = xo[x0 + RL];
-  float2 acc[RL][C];
-#pragma unroll
-  for (int r = 0; r < RL; r++) {
-#pragma unroll
-    for (int c = 0; c < C; c++) acc[r][c] = make_float2(0.f, 0.f);
-  }
-  // adds entry record `rec` to the thread's rows
-  auto add = [&](const float* rec) {
+  const int x0 = tid * RL, xp = x0 ^ (NC / 2);
+  const int e0 = a.ent_off[blk], e1 = a.ent_off[blk + 1];
+  // the runs of the rows at x; the loads unconditional, so that they are in
+  // flight together
+  auto runs = [&](int x, int (&l)[2], int (&h)[2]) {
+    const bool wrap = x < K - 1;
+    l[0] = xo[wrap ? NC - (K - 1) + x : NC];
+    h[0] = xo[NC];
+    l[1] = xo[wrap ? 0 : x - (K - 1)];
+    h[1] = xo[x + RL];
+  };
+  // adds entry record `rec` to the rows at x (accumulators ac)
+  auto add = [&](const float* rec, int x, float2 (&ac)[RL][C]) {
     const float4 tail = reinterpret_cast<const float4*>(rec)[XS / 4 + C / 4];
-    // x-tap slot of row x0: (x0 - z) + RL - 1, in [0, K + RL - 2] by the range
-    const int q = (x0 - __float_as_int(tail.z) + RL - 1) & (NC - 1);
+    // x-tap slot of row x: (x - z) + RL - 1, in [0, K + RL - 2] by the range
+    const int q = (x - __float_as_int(tail.z) + RL - 1) & (NC - 1);
     float px[RL];
 #pragma unroll
     for (int r = 0; r < RL; r++) px[r] = rec[q + r];
@@ -547,57 +552,112 @@ __device__ __forceinline__ void col_adj_body(const ColArgs& a, int bid) {
       const float fx = tail.x * px[r], fy = tail.y * px[r];
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        acc[r][c].x = __fmaf_rn(fx, py[c], acc[r][c].x);
-        acc[r][c].y = __fmaf_rn(fy, py[c], acc[r][c].y);
+        ac[r][c].x = __fmaf_rn(fx, py[c], ac[r][c].x);
+        ac[r][c].y = __fmaf_rn(fy, py[c], ac[r][c].y);
       }
     }
   };
-  const int e0 = a.ent_off[blk], e1 = a.ent_off[blk + 1];
-  for (int cb = e0; cb < e1; cb += CAP) {
-    const int cn = min(CAP, e1 - cb);
-    if (cb > e0) __syncthreads();   // the previous chunk has been consumed
-    for (int i = tid; i < cn; i += NT) {
-      const int e = cb + i;
-      float4* rec = reinterpret_cast<float4*>(stage + i * REC);
-      const float2* pxv = tile_px<K>(a, e0) + (e - e0);
-      float t[XS];
-#pragma unroll
-      for (int q = 0; q < XS; q++) t[q] = 0.f;
-#pragma unroll
-      for (int q = 0; q < K / 2; q++) {
-        const float2 v = pxv[q * (e1 - e0)];
-        t[RL - 1 + 2 * q] = v.x;
-        t[RL + 2 * q] = v.y;
-      }
-#pragma unroll
-      for (int q = 0; q < XS / 4; q++) rec[q] = make_float4(t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]);
-      const float4* pyv = tile_py<C>(a, e0) + (e - e0);
-#pragma unroll
-      for (int c4 = 0; c4 < C / 4; c4++) rec[XS / 4 + c4] = pyv[c4 * (e1 - e0)];
-      const float2 f = fw[e];
-      rec[XS / 4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
+  // adds the staged entries [lo, hi) of the chunk at cb (cn records) to the rows at x
+  auto add_range = [&](int lo, int hi, int cb, int cn, int x, float2 (&ac)[RL][C]) {
+    const int k1 = min(hi, cb + cn) - cb;
+    int k = max(lo, cb) - cb;
+    for (; k + 1 < k1; k += 2) {
+      add(stage + k * REC, x, ac);
+      add(stage + (k + 1) * REC, x, ac);
     }
-    __syncthreads();
-#pragma unroll
-    for (int run = 0; run < 2; run++) {
-      const int k1 = min(hi[run], cb + cn) - cb;
-      int k = max(lo[run], cb) - cb;
-      for (; k + 1 < k1; k += 2) {
-        add(stage + k * REC);
-        add(stage + (k + 1) * REC);
-      }
-      if (k < k1) add(stage + k * REC);
+    if (k < k1) add(stage + k * REC, x, ac);
+  };
+  FftTwiddles Tw;   // of the inverse x FFT, loaded while the tile is written
+  auto gather = [&](auto split_tag) {
+    constexpr bool SPLIT = decltype(split_tag)::value;
+    int lo[2], hi[2], lp[2], hp[2];
+    runs(x0, lo, hi);
+    if constexpr (SPLIT) {
+      runs(xp, lp, hp);
+      // own rows: the first ceil(n/2) entries; the partner's: the rest of its
+      const int n0 = hi[0] - lo[0], h = (n0 + hi[1] - lo[1] + 1) >> 1;
+      hi[0] = lo[0] + min(n0, h);
+      hi[1] = lo[1] + max(0, h - n0);
+      const int m0 = hp[0] - lp[0], g = (m0 + hp[1] - lp[1] + 1) >> 1;
+      lp[0] += min(m0, g);
+      lp[1] += max(0, g - m0);
     }
-  }
-  // twiddles of the inverse x FFT, in flight while the tile is written
-  FftTwiddles Tw;
-  load_twiddles<NC, 1>(Tw, tid / C, a.tw);
-  __syncthreads();
+    float2 acc[RL][C], accp[SPLIT ? RL : 1][C];
 #pragma unroll
-  for (int r = 0; r < RL; r++) {
+    for (int r = 0; r < RL; r++) {
 #pragma unroll
-    for (int c = 0; c < C; c++) tile[ColFftLay<C>::at(x0 + r) + c] = acc[r][c];
-  }
+      for (int c = 0; c < C; c++) acc[r][c] = make_float2(0.f, 0.f);
+    }
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+#pragma unroll
+        for (int c = 0; c < C; c++) accp[r][c] = make_float2(0.f, 0.f);
+      }
+    }
+    for (int cb = e0; cb < e1; cb += CAP) {
+      const int cn = min(CAP, e1 - cb);
+      if (cb > e0) __syncthreads();   // the previous chunk has been consumed
+      for (int i = tid; i < cn; i += NT) {
+        const int e = cb + i;
+        float4* rec = reinterpret_cast<float4*>(stage + i * REC);
+        const float2* pxv = tile_px<K>(a, e0) + (e - e0);
+        float t[XS];
+#pragma unroll
+        for (int q = 0; q < XS; q++) t[q] = 0.f;
+#pragma unroll
+        for (int q = 0; q < K / 2; q++) {
+          const float2 v = pxv[q * (e1 - e0)];
+          t[RL - 1 + 2 * q] = v.x;
+          t[RL + 2 * q] = v.y;
+        }
+#pragma unroll
+        for (int q = 0; q < XS / 4; q++) rec[q] = make_float4(t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]);
+        const float4* pyv = tile_py<C>(a, e0) + (e - e0);
+#pragma unroll
+        for (int c4 = 0; c4 < C / 4; c4++) rec[XS / 4 + c4] = pyv[c4 * (e1 - e0)];
+        const float2 f = fw[e];
+        rec[XS / 4 + C / 4] = make_float4(f.x, f.y, __int_as_float(a.ent[e].z), 0.f);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int run = 0; run < 2; run++) {
+        add_range(lo[run], hi[run], cb, cn, x0, acc);
+        if constexpr (SPLIT) add_range(lp[run], hp[run], cb, cn, xp, accp);
+      }
+    }
+    if constexpr (SPLIT) {
+      __syncthreads();   // the staged records are consumed
+      // the partner's second halves into the tile, then each thread adds
+      // its own first halves to them
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+#pragma unroll
+        for (int c = 0; c < C; c++) tile[ColFftLay<C>::at(xp + r) + c] = accp[r][c];
+      }
+      load_twiddles<NC, 1>(Tw, tid / C, a.tw);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+          float2& cell = tile[ColFftLay<C>::at(x0 + r) + c];
+          const float2 b = cell;
+          cell = make_float2(acc[r][c].x + b.x, acc[r][c].y + b.y);
+        }
+      }
+    } else {
+      load_twiddles<NC, 1>(Tw, tid / C, a.tw);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+#pragma unroll
+        for (int c = 0; c < C; c++) tile[ColFftLay<C>::at(x0 + r) + c] = acc[r][c];
+      }
+    }
+  };
+  if (e1 - e0 >= a.split_min) gather(std::true_type{});
+  else gather(std::false_type{});
   __syncthreads();
   // inverse x FFT of column c = tid % C; the last pass writes the N0 rows the
   // crop keeps (grid row x -> image row X = x + N0/2 mod NC) straight to global

@@ -1,6 +1,7 @@
 // Internal plan state of the MI355X ICON core (host side).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -80,6 +81,18 @@ inline void zero_band(const Bins& b, int* zlo, int* zhi) {
   while (hi < b.nblk && empty(hi)) hi++;
   *zlo = lo * b.C;
   *zhi = hi * b.C;
+}
+
+// Entry count from which a spread tile splits each row pair's entries
+// between two threads: three times the median count of the tiles that have
+// entries, so that only the few dense tiles around ky = 0 reach it.
+inline int split_min_entries(const Bins& b) {
+  std::vector<int> n;
+  for (int t = 0; t < b.nblk; t++)
+    if (b.off[t + 1] > b.off[t]) n.push_back(b.off[t + 1] - b.off[t]);
+  if (n.empty()) return 0x7fffffff;
+  std::nth_element(n.begin(), n.begin() + n.size() / 2, n.end());
+  return std::max(1, 3 * n[n.size() / 2]);
 }
 
 // Fills g; returns false (with a message) on an invalid geometry.
