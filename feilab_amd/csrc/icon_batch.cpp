@@ -75,6 +75,9 @@ struct icon_batch {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // item tables per size class
   std::vector<Items> row_fwd, row_adj, col, sino, reproj;
+  // the spread's items: those of col, by their tile's entry count, largest
+  // first across the class's series (each tile's blocks stay together)
+  std::vector<Items> col_adj;
   Items combine;
   // argument blocks, one per series (device copies written before each solve)
   std::vector<RowFwdArgs> h_rf;
@@ -116,6 +119,7 @@ int build_items(icon_batch* b) {
   free_items(b->row_fwd);
   free_items(b->row_adj);
   free_items(b->col);
+  free_items(b->col_adj);
   free_items(b->sino);
   free_items(b->reproj);
   if (b->combine.d) (void)hipFree(b->combine.d);
@@ -152,8 +156,22 @@ int build_items(icon_batch* b) {
     for (int k = 0; k < S; k++) rp.h.push_back(make_int2(i, k));
     for (int k = 0; k < S; k++) b->combine.h.push_back(make_int2(i, k));
   }
+  // the spread's order: a stable sort of each class's items by the entry
+  // count of the tile the block runs (the plan's own largest-first tile order
+  // maps a series' block k to tile bins.order[k / S])
+  for (const Items& c : b->col) {
+    Items a = c;
+    a.d = nullptr;
+    auto cnt = [&](const int2& q) {
+      const icon_plan* p = b->plans[q.x];
+      const int t = p->bins.order[q.y / b->S[q.x]];
+      return p->bins.off[t + 1] - p->bins.off[t];
+    };
+    std::stable_sort(a.h.begin(), a.h.end(), [&](const int2& u, const int2& v) { return cnt(u) > cnt(v); });
+    b->col_adj.push_back(std::move(a));
+  }
   int rc;
-  for (auto* v : {&b->row_fwd, &b->row_adj, &b->col, &b->sino, &b->reproj})
+  for (auto* v : {&b->row_fwd, &b->row_adj, &b->col, &b->col_adj, &b->sino, &b->reproj})
     for (auto& it : *v)
       if ((rc = upload_items(it))) return rc;
   if ((rc = upload_items(b->combine))) return rc;
@@ -234,7 +252,7 @@ int enqueue_batch(icon_batch* b, float* const* m, const float* const* m_init, bo
   const int iters = o.pre + o.icon + o.post;
   int rc;
   auto col_all = [&](bool adj) -> int {
-    for (auto& it : b->col)
+    for (auto& it : adj ? b->col_adj : b->col)
       if ((rc = ck(launch_col_multi(it.key0, it.key1, it.key2, adj, b->d_ca, it.d, it.n(), st),
                    adj ? "col_adj" : "col_fwd")))
         return rc;
@@ -303,6 +321,7 @@ int icon_batch_destroy(icon_batch* b) {
   free_items(b->row_fwd);
   free_items(b->row_adj);
   free_items(b->col);
+  free_items(b->col_adj);
   free_items(b->sino);
   free_items(b->reproj);
   if (b->combine.d) (void)hipFree(b->combine.d);
@@ -491,15 +510,18 @@ int icon_batch_autotune(icon_batch* b, int* col_blocks) {
       if ((rc = build_items(b))) return rc;
       fill_args(b, sino.data(), m.data(), rp.data(), rs.data(), al.data(), 0);
       if ((rc = upload_args(b->d_ca, b->h_ca, b->stream))) return rc;
-      const Items* it = nullptr;
+      const Items *it = nullptr, *ia = nullptr;
       for (auto& c : b->col)
         if (c.key0 == n0 && c.key1 == C && c.key2 == K) it = &c;
-      if (!it) continue;
+      for (auto& c : b->col_adj)
+        if (c.key0 == n0 && c.key1 == C && c.key2 == K) ia = &c;
+      if (!it || !ia) continue;
       double t = 0.0;
       for (int rep = 0; rep < 4 && rc == ICON_OK; rep++) {
         hipError_t e = hipEventRecord(e0, b->stream);
         for (int k = 0; k < 3 && e == hipSuccess; k++)
-          e = launch_col_multi(n0, C, K, k == 1, b->d_ca, it->d, it->n(), b->stream);
+          e = k == 1 ? launch_col_multi(n0, C, K, true, b->d_ca, ia->d, ia->n(), b->stream)
+                     : launch_col_multi(n0, C, K, false, b->d_ca, it->d, it->n(), b->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
