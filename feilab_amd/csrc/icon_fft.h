@@ -268,6 +268,43 @@ __device__ __forceinline__ void fft_from(float2 (&v)[16], float2* lds, int t, co
   }
 }
 
+// Twiddle slot P (the pass after the first that uses it) of T only, as
+// load_twiddles loads it.
+template <int N, int DIR, int P>
+__device__ __forceinline__ void load_twiddle_pass(FftTwiddles& T, int t, const float2* __restrict__ tw) {
+  constexpr int TH = N / 16;
+  if constexpr (P < twiddled_passes<N>()) {
+    const float4* p = reinterpret_cast<const float4*>(tw) + (DIR > 0 ? 16 * TH : 0) + t;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const float4 v = p[(P * 8 + i) * TH];
+      T.w[P][2 * i] = make_float2(v.x, v.y);
+      T.w[P][2 * i + 1] = make_float2(v.z, v.w);
+    }
+  }
+}
+
+// fft_from with each pass's twiddles loaded during the exchange before it,
+// so that only one pass's twiddles are held at a time (fewer registers than
+// loading them all up front).
+template <int N, typename L, int DIR, int NS, int P = 0, bool HALF = false>
+__device__ __forceinline__ void fft_from_lazy(float2 (&v)[16], float2* lds, int t, FftTwiddles& T,
+                                              const float2* __restrict__ tw) {
+  constexpr int RP = pass_radix<N, NS>();
+  if constexpr (HALF && NS == 1 && RP == 16) dft16_half<DIR>(v);
+  else pass_compute<N, RP, NS, DIR, P>(v, T);
+  if constexpr (NS * RP < N) {
+    constexpr int PN = NS > 1 ? P + 1 : 0;   // the next pass's twiddle slot
+    load_twiddle_pass<N, DIR, PN>(T, t, tw);
+    pass_store<N, RP, NS, L>(v, lds, t);
+    line_sync<0>();
+    constexpr int RN = pass_radix<N, NS * RP>();
+    pass_load<N, RN, L>(v, lds, t);
+    line_sync<0>();
+    fft_from_lazy<N, L, DIR, NS * RP, PN>(v, lds, t, T, tw);
+  }
+}
+
 // fft_regs of a zero-padded line: v[4..11] are zero and are not read.
 template <int N, typename L, int DIR>
 __device__ __forceinline__ void fft_regs_half(float2 (&v)[16], float2* lds, int t, const FftTwiddles& T) {

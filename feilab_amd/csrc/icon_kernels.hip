@@ -222,8 +222,9 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
   const bool valid = row < a.rows;
   const int s = valid ? row / a.N0 : 0;
   const int X = valid ? row % a.N0 : 0;
+  // twiddles loaded pass by pass during the FFTs' exchanges (one pass's at a
+  // time in registers)
   FftTwiddles Tw;
-  load_twiddles<NR, 1>(Tw, t, a.tw);
   // the row (row 0 for a team past the last row, its results dropped): all
   // 16 loads unconditional, so that they are in flight together.  Buffer
   // loads over the row: a column of the zero band gets an offset past the
@@ -240,7 +241,7 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
       v[r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
     }
   }
-  fft_regs<NR, RowLay, 1>(v, lds + team * kRowLds<NR>, t, Tw);
+  fft_from_lazy<NR, RowLay, 1, 1>(v, lds + team * kRowLds<NR>, t, Tw, a.tw);
   const int h1 = a.N1 >> 1;
   double acc = 0.0;
   float2 u[16];   // FWD: forward input, slot r holds element t + r T
@@ -259,8 +260,7 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
   }
   if constexpr (FWD) {
     // the inverse's last LDS exchange ended with a barrier
-    load_twiddles<NR, -1>(Tw, t, a.tw);
-    fft_regs_half<NR, RowLay, -1>(u, lds + team * kRowLds<NR>, t, Tw);
+    fft_from_lazy<NR, RowLay, -1, 1, 0, true>(u, lds + team * kRowLds<NR>, t, Tw, a.tw);
     if (valid) {
       float2* o = a.fwd_out + (size_t)row * NR;
       const unsigned zw = (unsigned)(a.zhi - a.zlo);
@@ -271,8 +271,9 @@ __device__ __forceinline__ void row_adj_body(const RowAdjArgs& a, int bid) {
   }
 }
 
-// At most 2 waves per SIMD requested (132 registers): capped at 128 the fused
-// forward pass spilled, and the kernel ran 2 % slower.
+// The launch bound asks for 2 waves per SIMD (up to 256 registers); with the
+// twiddles loaded pass by pass the kernel needs 112 and runs 4 waves per SIMD
+// (with all twiddles up front it needed 132, and capped at 128 it spilled).
 template <int NR, int RPB, bool FWD>
 __global__ __launch_bounds__(RPB * (NR / 16), 2) void k_row_adj(RowAdjArgs a) {
   row_adj_body<NR, RPB, FWD>(a, blockIdx.x);
