@@ -11,8 +11,8 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   (FP64) and reprojection, the residuals read back.  Each slice's final R is
   the target.
 * Learned leg: a U-Net trained on the device (its own BF16 MFMA kernels) on
-  (10-iteration, 200-iteration) pairs of 20480 slices of other seeded
-  phantoms; one step = 10 plain iterations, the U-Net warm start, k descent
+  (2-iteration, 200-iteration) pairs of 20480 slices of other seeded
+  phantoms; one step = 2 plain iterations, the U-Net warm start, k descent
   iterations with residuals and reprojection, the residuals read back, and
   plain ICON re-solving every slice whose R misses its target (the per-slice
   fallback, timed as part of the step).  k, with an output gain g of the
@@ -24,7 +24,7 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   0.005 below plain ICON's; of those g, the one with the lowest worst-slice
   residual.
 * Quality (not timed): per slice, relative L2 and FRC of the learned result
-  against plain ICON's and against the phantom, and the same for the 10 + k
+  against plain ICON's and against the phantom, and the same for the 2 + k
   iterations without the network.
 
 Prints one JSON line: value = learned-start slices/s, plus the plain leg,
@@ -509,7 +509,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--iters", default="20,160,20", help="pre,icon,post")
-    ap.add_argument("--start", type=int, default=10, help="plain iterations before the U-Net")
+    ap.add_argument("--start", type=int, default=2, help="plain iterations before the U-Net")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-pairs", type=int, default=20480, help="(start, final) training pairs, slices")
     ap.add_argument("--pair-chunk", type=int, default=256, help="slices per training phantom")
