@@ -11,8 +11,8 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   (FP64) and reprojection, the residuals read back.  Each slice's final R is
   the target.
 * Learned leg: a U-Net trained on the device (its own BF16 MFMA kernels) on
-  (2-iteration, 200-iteration) pairs of 20480 slices of other seeded
-  phantoms; one step = 2 plain iterations, the U-Net warm start, k descent
+  (10-iteration, 200-iteration) pairs of 20480 slices of other seeded
+  phantoms; one step = 10 plain iterations, the U-Net warm start, k descent
   iterations with residuals and reprojection, the residuals read back, and
   plain ICON re-solving every slice whose R misses its target (the per-slice
   fallback, timed as part of the step).  k, with an output gain g of the
@@ -24,8 +24,11 @@ Workload: every slice of a synthetic tilt series (256 slices of 512 x 512,
   0.005 below plain ICON's; of those g, the one with the lowest worst-slice
   residual.
 * Quality (not timed): per slice, relative L2 and FRC of the learned result
-  against plain ICON's and against the phantom, and the same for the 2 + k
+  against plain ICON's and against the phantom, and the same for the 10 + k
   iterations without the network.
+* Short-start variant (short_start_variant, not the headline): the same
+  learned leg with the network trained on, and started from, the
+  2-iteration iterate (--short-start).
 
 Prints one JSON line: value = learned-start slices/s, plus the plain leg,
 the roofline of the dominant ICON kernel (algorithmic bytes / HIP-event-timed
@@ -278,10 +281,11 @@ def rel_rows(a, b):
                                                                                         dtype=np.float64))
 
 
-def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, plain_value, timed):
-    """Trains the warm start, calibrates k on a held-out phantom, times the
-    learned solve with its fallback on the evaluated series and measures its
-    quality.  Returns (learned dict, unet_roofline dict, (value, ms_step) or None)."""
+def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, plain_value, timed, start):
+    """Trains the warm start on (start, final)-iteration pairs, calibrates k
+    on a held-out phantom, times the learned solve with its fallback on the
+    evaluated series and measures its quality.  Returns (learned dict,
+    unet_roofline dict, (value, ms_step) or None)."""
     import feilab_amd
     from feilab_amd import frc as F
     from feilab_amd import phantom, warmstart
@@ -295,17 +299,17 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
 
     # training pairs and the network
     log("training pairs: %d slices of seeded phantoms, (%d, %d) iterations, %d generator processes"
-        % (args.train_pairs, args.start, iters, workers))
+        % (args.train_pairs, start, iters, workers))
     t0 = time.perf_counter()
     x, t = warmstart.make_pairs(plan, pair_chunks(args.train_pairs, args.pair_chunk, 5001, workers),
-                                args.train_pairs, args.start, sched, log=log)
+                                args.train_pairs, start, sched, log=log)
     pairs_s = time.perf_counter() - t0
     log("pairs ready in %.1f s" % pairs_s)
     t0 = time.perf_counter()
     params, losses = warmstart.train_on_pairs(x, t, steps=args.train_steps, log=log)
     train_s = time.perf_counter() - t0
     del x, t
-    solver = warmstart.LearnedSolver(plan, params, S, start=args.start, schedule=sched, batch=args.unet_batch,
+    solver = warmstart.LearnedSolver(plan, params, S, start=start, schedule=sched, batch=args.unet_batch,
                                      overlap=bool(args.overlap))
 
     # continuation length (and output gain) from a held-out phantom of the training distribution
@@ -404,7 +408,7 @@ def learned_leg(args, plan, cfg, d_sino, sched, R_plain, m_plain, frc_plain, pla
                            no_network_frc=st(f_np)),
         frc_band=list(band),
         note="per slice; FRC = mean over rings %d..%d of calFRC's ring correlation of the two slices" % band)
-    learned = dict(start_iterations=args.start, continuation_iterations=k_star, output_gain=g_star,
+    learned = dict(start_iterations=start, continuation_iterations=k_star, output_gain=g_star,
                    slices_per_s=round(value, 3), speedup_vs_plain=round(value / plain_value, 3),
                    resid_mean=float(R_l.mean()), resid_max_over_target=round(float((R_l / R_plain).max()), 5),
                    fallback_slices_per_step=fallbacks[-1], fallback_slices_all_steps=fallbacks,
@@ -509,7 +513,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--iters", default="20,160,20", help="pre,icon,post")
-    ap.add_argument("--start", type=int, default=2, help="plain iterations before the U-Net")
+    ap.add_argument("--start", type=int, default=10, help="plain iterations before the U-Net")
+    ap.add_argument("--short-start", type=int, default=2,
+                    help="start length of a second learned leg reported as short_start_variant (0: none)")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-pairs", type=int, default=20480, help="(start, final) training pairs, slices")
     ap.add_argument("--pair-chunk", type=int, default=256, help="slices per training phantom")
@@ -667,9 +673,20 @@ def main():
     value, ms_step = plain_value, dt_plain / args.steps * 1e3
     if not args.no_learned:
         learned, unet_roofline, lv = learned_leg(args, plan, cfg, d_sino, (pre, icon_it, post), R_plain, m_plain,
-                                                 frc_plain, plain_value, timed)
+                                                 frc_plain, plain_value, timed, args.start)
         if lv is not None:
             value, ms_step = lv
+    # ---- the same learned leg from an earlier iterate (fewer plain iterations before the network)
+    short = None
+    if not args.no_learned and args.short_start and args.short_start != args.start:
+        log("learned leg from the %d-iteration iterate" % args.short_start)
+        ls, _, lsv = learned_leg(args, plan, cfg, d_sino, (pre, icon_it, post), R_plain, m_plain, frc_plain,
+                                 plain_value, timed, args.short_start)
+        short = dict(note="the learned leg with the U-Net trained on (%d, %d)-iteration pairs and started from "
+                          "the %d-iteration iterate; everything else as the headline leg"
+                          % (args.short_start, iters, args.short_start),
+                     slices_per_s=None if lsv is None else round(lsv[0], 3),
+                     ms_per_step=None if lsv is None else round(lsv[1], 3), learned=ls)
 
     cpu = None
     if not args.no_cpu:
@@ -715,6 +732,7 @@ def main():
         "iteration_roofline": iteration_roofline(plan, S, per_iter_ms),
         "unet_roofline": unet_roofline,
         "kb6_variant": kb6,
+        "short_start_variant": short,
     }
     print(json.dumps(out), flush=True)
 
